@@ -1,0 +1,171 @@
+/*
+ * kuramoto_amd — C ABI of the MI355X Kuramoto / Dormand–Prince 4(5) integrator.
+ *
+ * This is the in-process operator boundary that replaces the reference's
+ * integrator API (dp45.h:7-11, dp45_64.h:7-11) together with the model
+ * callbacks it drives (dydt kuramoto.cu:118, makecoupling kuramoto.cu:104,
+ * step_eval kuramoto.cu:164) and the `parameters` context
+ * (kuramoto.cu:14-40).  Differences from the reference API, by design:
+ *   - one opaque context per integrator instead of file-static globals
+ *     (dp45.cu:24-31), so several integrators can live in one process;
+ *   - the RHS is built in (fused HIP kernels) instead of a dydt callback;
+ *   - the step controller runs on the device, so stepping is a pull API
+ *     (kmt_advance + kmt_read_*) instead of a step_eval callback;
+ *   - every entry point returns a status code (KMT_OK = 0); the message of
+ *     the last failure on the calling thread is kmt_last_error().
+ * All array arguments are HOST pointers; sizes are element counts.  Real
+ * arrays use the context precision: float for KMT_F32, double for KMT_F64.
+ * Adjacency layout everywhere: row-major, element [i*ld + j] multiplies
+ * oscillator j in the equation of oscillator i (README.md:50, kuramoto.cu:113).
+ */
+#ifndef KURAMOTO_AMD_H
+#define KURAMOTO_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KMT_OK 0
+#define KMT_ERR_ARG (-1)     /* invalid argument or size */
+#define KMT_ERR_HIP (-2)     /* a HIP runtime call failed */
+#define KMT_ERR_STATE (-3)   /* call out of order (e.g. advance before begin) */
+#define KMT_ERR_NUMERIC (-4) /* non-finite error norm: integration stopped */
+#define KMT_ERR_MEMORY (-5)  /* not enough device memory */
+
+/* working precision: `kuramoto` (float) / `kuramoto_64` (double) */
+#define KMT_F32 0
+#define KMT_F64 1
+
+/* adjacency storage in HBM */
+#define KMT_ADJ_NATIVE 0 /* same type as the working precision */
+#define KMT_ADJ_BF16 2   /* bfloat16 with FP32 accumulation (KMT_F32 only) */
+#define KMT_ADJ_F16 3    /* IEEE half with FP32 accumulation (KMT_F32 only) */
+
+typedef struct kmt_ctx kmt_ctx;
+
+/* Snapshot of the device-resident integrator state, filled by kmt_advance. */
+typedef struct kmt_status {
+  double t;          /* integrator time (reference: *t in dp45_run) */
+  double h;          /* next step size */
+  double t_last;     /* start time of the last accepted step */
+  long accepted;     /* accepted steps (reference: parameters.steps) */
+  long rejected;     /* rejected attempts */
+  long attempts;     /* all attempts */
+  long rhs_evals;    /* RHS evaluations (6 per attempt + 1 per begin) */
+  int active;        /* 1 while t < t1 or outputs are pending */
+  int error;         /* nonzero after a non-finite error norm */
+  int rows_ready;    /* output rows waiting to be read (kmt_read_outputs) */
+  int times_ready;   /* accepted-step records waiting (kmt_read_times) */
+  int eval_i;        /* outputs emitted so far (index into t_eval) */
+  int pending;       /* outputs of the last accepted step not yet emitted */
+} kmt_status;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *kmt_last_error(void);
+
+/* Device query, used for the reference's memory check (kuramoto.cu:424-438). */
+int kmt_device_count(int *count);
+int kmt_mem_info(int device, size_t *free_bytes, size_t *total_bytes);
+
+/* ---- host-side generators (no GPU work) ------------------------------- */
+/* main()'s draw order: srand(seed); N initial-condition draws
+ * I/2*(2u-1); N frequency draws, normal or Cauchy (kuramoto.cu:416,
+ * :495-541; kuramoto_64.cu:492-538).  Bit-identical to the reference.
+ * A NULL output skips its draws, as the reference skips the
+ * initial-condition draws when -R reloads them. */
+int kmt_draws(int precision, long N, int seed, int normal, double freq, double init, void *theta0, void *omega);
+/* Low-rank factor X (N x K, row-major, double) from the library's
+ * Philox4x32-10 stream: Gaussian (Box–Muller) or Rademacher (+-1).
+ * Replaces makeu1 (kuramoto.cu:128-135) with a host generator. */
+int kmt_factor(long N, int K, uint32_t seed, int gaussian, double *X);
+/* Weights w so that M = X diag(w) X^T: volcano w_m = (-1)^m/N
+ * (makeu2, kuramoto.cu:137-143) or Gaussian w_m = -1/(N K). */
+int kmt_factor_weights(int precision, long N, int K, int gaussian, void *w);
+
+/* ---- context (parameters + dp45_init / dp45_destroy) ------------------ */
+/* dp45_init (dp45.cu:166-220) + the parameters struct (kuramoto.cu:14-40,
+ * :555).  c1 is the coupling coefficient of the RHS
+ * f_i = w_i + c1 (sin th_i C_i - cos th_i S_i) (kuramoto.cu:45). */
+int kmt_create(kmt_ctx **ctx, int device, int precision, long N, double atol, double rtol, int fixed_step, double c1);
+/* dp45_destroy (dp45.cu:222-251) plus the frees of main (kuramoto.cu:659-675). */
+int kmt_destroy(kmt_ctx *ctx);
+
+/* omegas upload (kuramoto.cu:552). */
+int kmt_set_omega(kmt_ctx *ctx, const void *omega);
+/* Phases, time and step size (dp45_init's y copy, dp45.cu:202; the -R
+ * restart of kuramoto.cu:475-483). */
+int kmt_set_state(kmt_ctx *ctx, const void *theta, double t, double h);
+/* Latest accepted state (phases at the last accepted step, t, h): what the
+ * reference writes to FILEBASEfs.dat after every step (kuramoto.cu:251-264). */
+int kmt_get_state(kmt_ctx *ctx, void *theta, double *t, double *h);
+
+/* Dense adjacency from host memory (the -R adj.dat path, kuramoto.cu:559-566).
+ * M: N x N values of type `src_precision` with row stride ld (>= N).
+ * storage: KMT_ADJ_NATIVE / _BF16 / _F16 (reduced storage uses a power-of-two
+ * scale chosen from max|M| so the stored values are O(1)). */
+int kmt_set_adjacency(kmt_ctx *ctx, const void *M, int src_precision, long ld, int storage);
+/* Low-rank factor for the adjacency: X (N x K, row-major, context
+ * precision) and weights w (K).  Used by kmt_build_adjacency and -A. */
+int kmt_set_factor(kmt_ctx *ctx, const void *X, const void *w, int K);
+/* Build the dense adjacency on the device from the factor (getadj,
+ * kuramoto.cu:145-161): M_ij = sum_m fma(w_m, X_im X_jm, .) in the working
+ * precision, stored as `storage`.  Exactly symmetric. */
+int kmt_build_adjacency(kmt_ctx *ctx, int storage);
+/* -A: couple through the factor (O(N K) per RHS) instead of a dense matrix
+ * (makecoupling's m1/m2 branch, kuramoto.cu:62-108). */
+int kmt_use_factored(kmt_ctx *ctx, int on);
+/* Read back rows [i0, i1) of the dense adjacency (dequantized, context
+ * precision, N values per row) — the adj.dat writer (kuramoto.cu:570-578). */
+int kmt_get_adjacency_rows(kmt_ctx *ctx, long i0, long i1, void *out);
+/* Storage scale: M = scale * stored (1 for native storage). */
+int kmt_get_adjacency_scale(kmt_ctx *ctx, double *scale);
+
+/* ---- model evaluations ------------------------------------------------- */
+/* dydt (kuramoto.cu:118-126): f = RHS(theta); optionally the couplings
+ * (C_i, S_i) interleaved, 2N values (makecoupling, kuramoto.cu:104-116). */
+int kmt_rhs(kmt_ctx *ctx, const void *theta, void *f, void *couplings);
+/* Order parameter r = |mean exp(i theta)| (kuramoto.cu:216-220, :592-597). */
+int kmt_order(kmt_ctx *ctx, const void *theta, double *r);
+
+/* ---- integration (dp45_run / dp45_step / dp45_eval / step_eval) ------- */
+/* Start a run (dp45_run's first dydt, dp45.cu:150-151): output grid
+ * t_eval[n_eval] (kuramoto.cu:543-550), end time t1, whether interpolated
+ * phase rows are kept for kmt_read_outputs (dense >= 1), and the number of
+ * output rows the device ring holds before the integrator pauses. */
+int kmt_begin(kmt_ctx *ctx, const double *t_eval, int n_eval, double t1, int keep_rows, int ring_rows);
+/* Run up to max_attempts step attempts on the device (the while loop of
+ * dp45_run, dp45.cu:153-162), then synchronise and report the state. */
+int kmt_advance(kmt_ctx *ctx, int max_attempts, kmt_status *st);
+/* Drain the output ring: up to max_rows rows of interpolated phases
+ * (N values each, if keep_rows) and order parameters (step_eval,
+ * kuramoto.cu:212-249).  Either pointer may be NULL. */
+int kmt_read_outputs(kmt_ctx *ctx, int max_rows, void *theta_rows, void *r, int *n_rows);
+/* Drain the accepted-step log: t and h after each accepted step
+ * (times.dat, kuramoto.cu:181-188, and the verbose line's h). */
+int kmt_read_times(kmt_ctx *ctx, int max_n, void *t, void *h, int *n);
+/* One attempt with a host synchronisation (dp45_step, dp45.cu:87-141). */
+int kmt_step(kmt_ctx *ctx, int *accepted, double *t, double *h);
+/* Dense output over the last accepted step at t_eval (dp45_eval,
+ * dp45.cu:143-146).  After the run this is the final-state interpolation of
+ * kuramoto.cu:632 (with k1 already replaced by k7, as in the reference). */
+int kmt_eval(kmt_ctx *ctx, double t_eval, void *theta);
+/* Time the dominant kernel: `reps` launches of the fused stage kernel on
+ * the library's stream, bracketed by HIP events; average ms per launch. */
+int kmt_time_stage_kernel(kmt_ctx *ctx, int reps, double *avg_ms);
+/* Time `attempts` step attempts on the library's stream with HIP events;
+ * reports wall ms for the whole region and the average ms of the stage
+ * kernel launches inside it (events around each stage launch). */
+int kmt_time_attempts(kmt_ctx *ctx, int attempts, double *region_ms, double *stage_avg_ms, kmt_status *st);
+
+/* Sizes the library chose (for roofline bookkeeping): padded row stride of
+ * the adjacency in elements, bytes per stored element, rows per workgroup
+ * of the stage kernel. */
+int kmt_layout(kmt_ctx *ctx, long *lda, int *elem_bytes, int *rows_per_block);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,337 @@
+"""kuramoto_amd — Python bindings (ctypes) of the MI355X Kuramoto/DP45 library.
+
+The library itself is C++/HIP behind the C ABI of ``include/kuramoto_amd.h``
+(built into ``kuramoto_amd/lib/libkuramoto_amd.so``); this module only
+declares that ABI for Python callers (tests, the benchmark).  It never falls
+back to a CPU implementation: if the shared library is missing it raises.
+
+Reference counterparts: the ``parameters`` context and callbacks of
+kuramoto.cu:14-265 and the integrator API of dp45.h:7-11.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libkuramoto_amd.so")
+BIN_DIR = os.path.join(_HERE, "bin")
+HEADER = os.path.join(os.path.dirname(_HERE), "include", "kuramoto_amd.h")
+
+F32, F64 = 0, 1
+ADJ_NATIVE, ADJ_BF16, ADJ_F16 = 0, 2, 3
+STORAGE = {"native": ADJ_NATIVE, "bf16": ADJ_BF16, "f16": ADJ_F16}
+
+ERRORS = {-1: "KMT_ERR_ARG", -2: "KMT_ERR_HIP", -3: "KMT_ERR_STATE", -4: "KMT_ERR_NUMERIC", -5: "KMT_ERR_MEMORY"}
+
+
+class KmtError(RuntimeError):
+    def __init__(self, code: int, what: str, msg: str):
+        super().__init__(f"{what}: {ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Status(ctypes.Structure):
+    _fields_ = [
+        ("t", ctypes.c_double),
+        ("h", ctypes.c_double),
+        ("t_last", ctypes.c_double),
+        ("accepted", ctypes.c_long),
+        ("rejected", ctypes.c_long),
+        ("attempts", ctypes.c_long),
+        ("rhs_evals", ctypes.c_long),
+        ("active", ctypes.c_int),
+        ("error", ctypes.c_int),
+        ("rows_ready", ctypes.c_int),
+        ("times_ready", ctypes.c_int),
+        ("eval_i", ctypes.c_int),
+        ("pending", ctypes.c_int),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+_P = ctypes.c_void_p
+_L = ctypes.c_long
+_I = ctypes.c_int
+_D = ctypes.c_double
+
+_SIGNATURES = {
+    "kmt_last_error": ([], ctypes.c_char_p),
+    "kmt_device_count": ([_P], _I),
+    "kmt_mem_info": ([_I, _P, _P], _I),
+    "kmt_draws": ([_I, _L, _I, _I, _D, _D, _P, _P], _I),
+    "kmt_factor": ([_L, _I, ctypes.c_uint32, _I, _P], _I),
+    "kmt_factor_weights": ([_I, _L, _I, _I, _P], _I),
+    "kmt_create": ([_P, _I, _I, _L, _D, _D, _I, _D], _I),
+    "kmt_destroy": ([_P], _I),
+    "kmt_set_omega": ([_P, _P], _I),
+    "kmt_set_state": ([_P, _P, _D, _D], _I),
+    "kmt_get_state": ([_P, _P, _P, _P], _I),
+    "kmt_set_adjacency": ([_P, _P, _I, _L, _I], _I),
+    "kmt_set_factor": ([_P, _P, _P, _I], _I),
+    "kmt_build_adjacency": ([_P, _I], _I),
+    "kmt_use_factored": ([_P, _I], _I),
+    "kmt_get_adjacency_rows": ([_P, _L, _L, _P], _I),
+    "kmt_get_adjacency_scale": ([_P, _P], _I),
+    "kmt_rhs": ([_P, _P, _P, _P], _I),
+    "kmt_order": ([_P, _P, _P], _I),
+    "kmt_begin": ([_P, _P, _I, _D, _I, _I], _I),
+    "kmt_advance": ([_P, _I, _P], _I),
+    "kmt_read_outputs": ([_P, _I, _P, _P, _P], _I),
+    "kmt_read_times": ([_P, _I, _P, _P, _P], _I),
+    "kmt_step": ([_P, _P, _P, _P], _I),
+    "kmt_eval": ([_P, _D, _P], _I),
+    "kmt_time_stage_kernel": ([_P, _I, _P], _I),
+    "kmt_time_attempts": ([_P, _I, _P, _P, _P], _I),
+    "kmt_layout": ([_P, _P, _P, _P], _I),
+}
+
+
+def load():
+    """Load the shared library (raises ImportError when it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() or make -C kuramoto_amd/csrc")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (args, res) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = lib
+    return lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(_P)
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise KmtError(rc, what, load().kmt_last_error().decode())
+
+
+def _dtype(precision):
+    return np.float32 if precision == F32 else np.float64
+
+
+def device_count() -> int:
+    n = ctypes.c_int(0)
+    _check(load().kmt_device_count(ctypes.byref(n)), "kmt_device_count")
+    return n.value
+
+
+def draws(precision, N, seed, normal=False, freq=1.0, init=0.0, ics=True, freqs=True):
+    """main()'s rand() draws: (theta0, omega) (kuramoto.cu:416, :495-541)."""
+    dt = _dtype(precision)
+    th = np.zeros(N, dt) if ics else None
+    om = np.zeros(N, dt) if freqs else None
+    _check(load().kmt_draws(precision, N, seed, int(normal), freq, init, _ptr(th), _ptr(om)), "kmt_draws")
+    return th, om
+
+
+def factor(N, K, seed, gaussian=True):
+    X = np.zeros((N, K), np.float64)
+    _check(load().kmt_factor(N, K, seed, int(gaussian), _ptr(X) if X.size else _ptr(np.zeros(1))), "kmt_factor")
+    return X
+
+
+def factor_weights(precision, N, K, gaussian=True):
+    w = np.zeros(max(K, 1), _dtype(precision))
+    _check(load().kmt_factor_weights(precision, N, K, int(gaussian), _ptr(w)), "kmt_factor_weights")
+    return w[:K]
+
+
+class Integrator:
+    """One integrator context on one GPU (kmt_create ... kmt_destroy)."""
+
+    def __init__(self, N, precision=F32, atol=1e-6, rtol=0.0, fixed=False, c1=3.0, device=0):
+        self.lib = load()
+        self.N = int(N)
+        self.precision = precision
+        self.dtype = _dtype(precision)
+        self.ctx = ctypes.c_void_p()
+        _check(self.lib.kmt_create(ctypes.byref(self.ctx), device, precision, self.N, atol, rtol, int(fixed), c1),
+               "kmt_create")
+
+    def close(self):
+        if self.ctx:
+            self.lib.kmt_destroy(self.ctx)
+            self.ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _vec(self, a, n=None):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        if n is not None and a.size != n:
+            raise ValueError(f"expected {n} values, got {a.size}")
+        return a
+
+    # ---- inputs ----
+    def set_omega(self, omega):
+        om = self._vec(omega, self.N)
+        _check(self.lib.kmt_set_omega(self.ctx, _ptr(om)), "kmt_set_omega")
+
+    def set_state(self, theta, t=0.0, h=1e-2):
+        th = self._vec(theta, self.N)
+        _check(self.lib.kmt_set_state(self.ctx, _ptr(th), t, h), "kmt_set_state")
+
+    def get_state(self):
+        th = np.zeros(self.N, self.dtype)
+        t, h = ctypes.c_double(), ctypes.c_double()
+        _check(self.lib.kmt_get_state(self.ctx, _ptr(th), ctypes.byref(t), ctypes.byref(h)), "kmt_get_state")
+        return th, t.value, h.value
+
+    def set_adjacency(self, M, storage="native"):
+        M = np.ascontiguousarray(M, dtype=self.dtype)
+        if M.shape != (self.N, self.N):
+            raise ValueError("adjacency must be N x N")
+        _check(self.lib.kmt_set_adjacency(self.ctx, _ptr(M), self.precision, self.N, STORAGE[storage]),
+               "kmt_set_adjacency")
+
+    def set_factor(self, X, w):
+        X = np.ascontiguousarray(X, dtype=self.dtype)
+        w = np.ascontiguousarray(w, dtype=self.dtype)
+        K = w.size
+        if X.shape != (self.N, K):
+            raise ValueError("factor must be N x K")
+        _check(self.lib.kmt_set_factor(self.ctx, _ptr(X) if K else None, _ptr(w) if K else None, K),
+               "kmt_set_factor")
+
+    def build_adjacency(self, storage="native"):
+        _check(self.lib.kmt_build_adjacency(self.ctx, STORAGE[storage]), "kmt_build_adjacency")
+
+    def use_factored(self, on=True):
+        _check(self.lib.kmt_use_factored(self.ctx, int(on)), "kmt_use_factored")
+
+    def adjacency_rows(self, i0, i1):
+        out = np.zeros((i1 - i0, self.N), self.dtype)
+        _check(self.lib.kmt_get_adjacency_rows(self.ctx, i0, i1, _ptr(out)), "kmt_get_adjacency_rows")
+        return out
+
+    def adjacency_scale(self):
+        s = ctypes.c_double()
+        _check(self.lib.kmt_get_adjacency_scale(self.ctx, ctypes.byref(s)), "kmt_get_adjacency_scale")
+        return s.value
+
+    # ---- evaluations ----
+    def rhs(self, theta, couplings=False):
+        th = self._vec(theta, self.N)
+        f = np.zeros(self.N, self.dtype)
+        cp = np.zeros(2 * self.N, self.dtype) if couplings else None
+        _check(self.lib.kmt_rhs(self.ctx, _ptr(th), _ptr(f), _ptr(cp)), "kmt_rhs")
+        return (f, cp) if couplings else f
+
+    def couplings(self, theta):
+        th = self._vec(theta, self.N)
+        cp = np.zeros(2 * self.N, self.dtype)
+        _check(self.lib.kmt_rhs(self.ctx, _ptr(th), None, _ptr(cp)), "kmt_rhs")
+        return cp
+
+    def order(self, theta):
+        th = self._vec(theta, self.N)
+        r = ctypes.c_double()
+        _check(self.lib.kmt_order(self.ctx, _ptr(th), ctypes.byref(r)), "kmt_order")
+        return r.value
+
+    # ---- integration ----
+    def begin(self, t_eval, t1, keep_rows=True, ring_rows=64):
+        te = np.ascontiguousarray(t_eval, dtype=np.float64)
+        self.keep_rows = bool(keep_rows)
+        self.ring_rows = int(ring_rows)
+        _check(self.lib.kmt_begin(self.ctx, _ptr(te) if te.size else None, te.size, t1, int(keep_rows), ring_rows),
+               "kmt_begin")
+
+    def advance(self, max_attempts):
+        st = Status()
+        _check(self.lib.kmt_advance(self.ctx, max_attempts, ctypes.byref(st)), "kmt_advance")
+        return st
+
+    def read_outputs(self, max_rows=None):
+        n_max = self.ring_rows if max_rows is None else max_rows
+        rows = np.zeros((n_max, self.N), self.dtype) if self.keep_rows else None
+        r = np.zeros(n_max, self.dtype)
+        n = ctypes.c_int()
+        _check(self.lib.kmt_read_outputs(self.ctx, n_max, _ptr(rows), _ptr(r), ctypes.byref(n)), "kmt_read_outputs")
+        return (rows[: n.value] if rows is not None else None), r[: n.value]
+
+    def read_times(self, max_n=4096):
+        t = np.zeros(max_n, self.dtype)
+        h = np.zeros(max_n, self.dtype)
+        n = ctypes.c_int()
+        _check(self.lib.kmt_read_times(self.ctx, max_n, _ptr(t), _ptr(h), ctypes.byref(n)), "kmt_read_times")
+        return t[: n.value], h[: n.value]
+
+    def step(self):
+        acc = ctypes.c_int()
+        t, h = ctypes.c_double(), ctypes.c_double()
+        _check(self.lib.kmt_step(self.ctx, ctypes.byref(acc), ctypes.byref(t), ctypes.byref(h)), "kmt_step")
+        return bool(acc.value), t.value, h.value
+
+    def eval(self, t_eval):
+        out = np.zeros(self.N, self.dtype)
+        _check(self.lib.kmt_eval(self.ctx, t_eval, _ptr(out)), "kmt_eval")
+        return out
+
+    def time_stage_kernel(self, reps=10):
+        ms = ctypes.c_double()
+        _check(self.lib.kmt_time_stage_kernel(self.ctx, reps, ctypes.byref(ms)), "kmt_time_stage_kernel")
+        return ms.value
+
+    def time_attempts(self, attempts):
+        region, stage = ctypes.c_double(), ctypes.c_double()
+        st = Status()
+        _check(self.lib.kmt_time_attempts(self.ctx, attempts, ctypes.byref(region), ctypes.byref(stage),
+                                          ctypes.byref(st)), "kmt_time_attempts")
+        return region.value, stage.value, st
+
+    def layout(self):
+        lda, eb, rows = ctypes.c_long(), ctypes.c_int(), ctypes.c_int()
+        _check(self.lib.kmt_layout(self.ctx, ctypes.byref(lda), ctypes.byref(eb), ctypes.byref(rows)), "kmt_layout")
+        return lda.value, eb.value, rows.value
+
+    def run(self, t_eval, t1, keep_rows=True, ring_rows=64, chunk=32, max_attempts=None):
+        """dp45_run + step_eval's output walk: returns a dict with the phase
+        rows at t_eval (if kept), r at t_eval, accepted-step times and h, and
+        the final status."""
+        self.begin(t_eval, t1, keep_rows, ring_rows)
+        rows, rs, ts, hs = [], [], [], []
+        done = 0
+        while True:
+            n = chunk if max_attempts is None else min(chunk, max_attempts - done)
+            st = self.advance(n)
+            done += n
+            t, h = self.read_times()
+            ts.append(t)
+            hs.append(h)
+            r_rows, r = self.read_outputs()
+            if r_rows is not None:
+                rows.append(r_rows)
+            rs.append(r)
+            if not st.active or (max_attempts is not None and done >= max_attempts):
+                break
+        out = {
+            "times": np.concatenate(ts) if ts else np.zeros(0, self.dtype),
+            "hs": np.concatenate(hs) if hs else np.zeros(0, self.dtype),
+            "r": np.concatenate(rs) if rs else np.zeros(0, self.dtype),
+            "status": st,
+        }
+        if keep_rows:
+            out["rows"] = np.concatenate(rows) if rows else np.zeros((0, self.N), self.dtype)
+        return out

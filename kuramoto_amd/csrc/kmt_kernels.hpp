@@ -1,0 +1,747 @@
+// HIP device code of the integrator (gfx950 / CDNA4, wave64).
+//
+// One DP45 attempt is 9 launches on one stream (captured in a hipGraph):
+//   stage_kernel<2..7>  A streamed from HBM once per stage for both cos and
+//                       sin; the epilogue writes k_s and the next stage's
+//                       (cos, sin) of y + h*sum a_{s+1,j} k_j (stage 6 also
+//                       keeps the 5th-order solution; stage 7 forms the
+//                       embedded error and a per-workgroup sum of squares)
+//   control_kernel      error norm, accept/reject, step-size update,
+//                       accepted-step log, output times crossed
+//   emit_commit_kernel  dense output at the crossed output times + order
+//                       parameter partial sums, then commit (ylast <- y,
+//                       y <- y5, k1 <- k7) and the next attempt's stage-2
+//                       (cos, sin)
+//   finalize_kernel     order parameter of each emitted row, ring and
+//                       output bookkeeping, end-of-run detection
+// Reference counterparts: dp45.cu:34-164 (stages, error, controller, FSAL,
+// interpolation) and kuramoto.cu:42-55, :104-126, :164-249 (RHS, order
+// parameter, output grid).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace kmt {
+
+constexpr int kBlock = 256;  // threads per workgroup (4 waves of 64)
+constexpr int kWave = 64;
+
+// ---------------------------------------------------------------------
+// Dormand–Prince 4(5) coefficients (dp45.cu:5-21).  Each entry is the same
+// double expression as the reference, rounded to the working precision.
+// ---------------------------------------------------------------------
+namespace tab {
+constexpr double A[6][6] = {
+    {1.0 / 5, 0, 0, 0, 0, 0},
+    {3.0 / 40, 9.0 / 40, 0, 0, 0, 0},
+    {44.0 / 45, -56.0 / 15, 32.0 / 9, 0, 0, 0},
+    {19372.0 / 6561, -25360.0 / 2187, 64448.0 / 6561, -212.0 / 729, 0, 0},
+    {9017.0 / 3168, -355.0 / 33, 46732.0 / 5247, 49.0 / 176, -5103.0 / 18656, 0},
+    {35.0 / 384, 0, 500.0 / 1113, 125.0 / 192, -2187.0 / 6784, 11.0 / 84}};
+constexpr double E[7] = {-71.0 / 57600, 0, 71.0 / 16695, -71.0 / 1920, 17253.0 / 339200, -22.0 / 525, 1.0 / 40};
+constexpr double P[7][4] = {
+    {1.0, -8048581381.0 / 2820520608, 8663915743.0 / 2820520608, -12715105075.0 / 11282082432},
+    {0, 0, 0, 0},
+    {0, 131558114200.0 / 32700410799, -68118460800.0 / 10900136933, 87487479700.0 / 32700410799},
+    {0, -1754552775.0 / 470086768, 14199869525.0 / 1410260304, -10690763975.0 / 1880347072},
+    {0, 127303824393.0 / 49829197408, -318862633887.0 / 49829197408, 701980252875.0 / 199316789632},
+    {0, -282668133.0 / 205662961, 2019193451.0 / 616988883, -1453857185.0 / 822651844},
+    {0, 40617522.0 / 29380423, -110615467.0 / 29380423, 69997945.0 / 29380423}};
+}  // namespace tab
+
+// ---------------------------------------------------------------------
+// Device-resident integrator state (one per context).
+// ---------------------------------------------------------------------
+template <class R> struct State {
+  R t, h, t_last, t1;
+  R atl, rtl;
+  int active;          // run in progress
+  int commit_pending;  // accepted step awaiting output emission + commit
+  int pend_lo, pend_hi;
+  int eval_i, n_eval;
+  int ring_count, ring_cap;
+  int times_count, times_cap;
+  int fixed;
+  int last_accept;
+  int error;
+  int pad_;
+  long long accepted, rejected, attempts, rhs_evals;
+  double norm;
+};
+
+// Storage tags for the adjacency.
+struct bf16_t { uint16_t bits; };
+struct f16_t { uint16_t bits; };
+
+template <class S> struct Store;
+template <> struct Store<float> { static constexpr int VEC = 4; using Acc = float; };
+template <> struct Store<double> { static constexpr int VEC = 2; using Acc = double; };
+template <> struct Store<bf16_t> { static constexpr int VEC = 8; using Acc = float; };
+template <> struct Store<f16_t> { static constexpr int VEC = 8; using Acc = float; };
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float bf16_lo(uint32_t w) { return __builtin_bit_cast(float, w << 16); }
+__device__ __forceinline__ float bf16_hi(uint32_t w) { return __builtin_bit_cast(float, w & 0xFFFF0000u); }
+__device__ __forceinline__ float f16_lo(uint32_t w) { return (float)__builtin_bit_cast(_Float16, (uint16_t)(w & 0xFFFFu)); }
+__device__ __forceinline__ float f16_hi(uint32_t w) { return (float)__builtin_bit_cast(_Float16, (uint16_t)(w >> 16)); }
+
+// 16 bytes of the adjacency -> VEC accumulator-typed values.
+template <class S> __device__ __forceinline__ void unpack(const u32x4 v, typename Store<S>::Acc *out);
+template <> __device__ __forceinline__ void unpack<float>(const u32x4 v, float *o) {
+  o[0] = __builtin_bit_cast(float, v[0]); o[1] = __builtin_bit_cast(float, v[1]);
+  o[2] = __builtin_bit_cast(float, v[2]); o[3] = __builtin_bit_cast(float, v[3]);
+}
+template <> __device__ __forceinline__ void unpack<double>(const u32x4 v, double *o) {
+  o[0] = __builtin_bit_cast(double, ((uint64_t)v[1] << 32) | v[0]);
+  o[1] = __builtin_bit_cast(double, ((uint64_t)v[3] << 32) | v[2]);
+}
+template <> __device__ __forceinline__ void unpack<bf16_t>(const u32x4 v, float *o) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q) { o[2 * q] = bf16_lo(v[q]); o[2 * q + 1] = bf16_hi(v[q]); }
+}
+template <> __device__ __forceinline__ void unpack<f16_t>(const u32x4 v, float *o) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q) { o[2 * q] = f16_lo(v[q]); o[2 * q + 1] = f16_hi(v[q]); }
+}
+
+// VEC consecutive reals (16- or 32-byte aligned).
+template <class R, int VEC> __device__ __forceinline__ void load_reals(const R *p, R *o) {
+  constexpr int W = (int)(VEC * sizeof(R) / 16);
+  static_assert(W >= 1, "vector load width");
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    const u32x4 v = *reinterpret_cast<const u32x4 *>(reinterpret_cast<const char *>(p) + 16 * w);
+    if constexpr (sizeof(R) == 4) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) o[4 * w + q] = __builtin_bit_cast(float, v[q]);
+    } else {
+      o[2 * w] = __builtin_bit_cast(double, ((uint64_t)v[1] << 32) | v[0]);
+      o[2 * w + 1] = __builtin_bit_cast(double, ((uint64_t)v[3] << 32) | v[2]);
+    }
+  }
+}
+
+// Working-precision math (float -> the f-suffixed device functions).
+template <class R> __device__ __forceinline__ R rcos(R x) {
+  if constexpr (sizeof(R) == 4) return cosf(x); else return cos(x);
+}
+template <class R> __device__ __forceinline__ R rsin(R x) {
+  if constexpr (sizeof(R) == 4) return sinf(x); else return sin(x);
+}
+template <class R> __device__ __forceinline__ R rabs(R x) {
+  if constexpr (sizeof(R) == 4) return fabsf(x); else return fabs(x);
+}
+template <class R> __device__ __forceinline__ R rmax(R a, R b) {
+  if constexpr (sizeof(R) == 4) return fmaxf(a, b); else return fmax(a, b);
+}
+
+template <class T> __device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+// Deterministic block sum of one double per thread (fixed tree order).
+__device__ __forceinline__ double block_sum(double v, double *sh) {
+  const int tid = threadIdx.x;
+  sh[tid] = v;
+  __syncthreads();
+#pragma unroll
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if (tid < off) sh[tid] += sh[tid + off];
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// ---------------------------------------------------------------------
+// Shared epilogue of every RHS evaluation.
+// ---------------------------------------------------------------------
+template <class R> struct Vecs {
+  const R *cs_in;  // cos at [0, ldv), sin at [ldv, 2 ldv)
+  R *cs_out;
+  long ldv;        // padded vector length
+  long N;
+  const R *omega;
+  R *y;
+  R *k[7];
+  R *ynew;         // 5th-order solution of the current attempt
+  R *f_out;        // KIND_RHS output
+  R *cs_pairs;     // interleaved (C_i, S_i), or nullptr
+  R c1;
+  State<R> *st;
+  double *err_part;  // one per workgroup (stage 7)
+  int check_state;   // stage kernels: skip unless an attempt is live
+};
+
+constexpr int KIND_RHS = 0;       // f = RHS(theta), optional couplings
+constexpr int KIND_COUPLING = 1;  // couplings only
+// KIND 2..7 = DP45 stage s
+
+// kuramoto.cu:45: f = w + c1 (sin th C - cos th S), contracted as
+// fma(c1, fma(sin, C, -(cos S)), w).
+template <class R> __device__ __forceinline__ R rhs_value(R om, R c1, R cs, R sn, R C, R S) {
+  return __builtin_fma(c1, __builtin_fma(sn, C, -(cs * S)), om);
+}
+
+// Row epilogue for oscillator i.  Returns the squared scaled error of
+// stage 7 (0 otherwise).  Stage combinations follow dp45.cu:34-69 with the
+// contraction acc = k1*a0; acc = fma(k_j, a_j, acc); y + h*acc -> fma(acc, h, y).
+template <class R, int KIND>
+__device__ __forceinline__ double row_epilogue(const Vecs<R> &a, long i, R C, R S, R h) {
+  const R cs = a.cs_in[i], sn = a.cs_in[a.ldv + i];
+  const R f = rhs_value<R>(a.omega[i], a.c1, cs, sn, C, S);
+  if constexpr (KIND == KIND_RHS) {
+    a.f_out[i] = f;
+    if (a.cs_pairs) { a.cs_pairs[2 * i] = C; a.cs_pairs[2 * i + 1] = S; }
+    return 0.0;
+  } else if constexpr (KIND == KIND_COUPLING) {
+    a.cs_pairs[2 * i] = C;
+    a.cs_pairs[2 * i + 1] = S;
+    return 0.0;
+  } else {
+    constexpr int s = KIND;  // stage 2..7, k_s = f
+    a.k[s - 1][i] = f;
+    if constexpr (s < 7) {
+      // coefficients of stage s+1 over k_1..k_s
+      R acc = a.k[0][i] * (R)tab::A[s - 1][0];
+#pragma unroll
+      for (int j = 1; j < s; ++j) {
+        const R kj = (j == s - 1) ? f : a.k[j][i];
+        acc = __builtin_fma(kj, (R)tab::A[s - 1][j], acc);
+      }
+      const R yt = __builtin_fma(acc, h, a.y[i]);
+      if constexpr (s + 1 == 7) a.ynew[i] = yt;
+      a.cs_out[i] = rcos<R>(yt);
+      a.cs_out[a.ldv + i] = rsin<R>(yt);
+      return 0.0;
+    } else {
+      // embedded error, dp45.cu:79-84
+      R acc = a.k[0][i] * (R)tab::E[0];
+#pragma unroll
+      for (int j = 1; j < 7; ++j) {
+        const R kj = (j == 6) ? f : a.k[j][i];
+        acc = __builtin_fma(kj, (R)tab::E[j], acc);
+      }
+      const R st = a.st->rtl;
+      const R num = h * acc;
+      const R den = __builtin_fma(st, rmax<R>(rabs<R>(a.y[i]), rabs<R>(a.ynew[i])), a.st->atl);
+      const R e = num / den;
+      return (double)e * (double)e;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------
+// Fused gemv-pair stage kernel (dense adjacency).
+// Each workgroup owns ROWS consecutive rows; all 256 threads sweep the
+// columns with 16-byte loads (two chunks in flight per row), reuse each
+// (cos, sin) chunk for all ROWS rows from registers, then reduce across
+// the workgroup (wave shuffles + LDS) in a fixed order.
+// ---------------------------------------------------------------------
+template <class R, class S> struct DenseArgs {
+  const S *A;
+  long lda;     // row stride (elements), multiple of 64
+  R mscale;     // M = mscale * stored
+  Vecs<R> v;
+};
+
+template <class R, class S, int ROWS, int KIND, bool NT>
+__global__ __launch_bounds__(kBlock) void stage_kernel(const DenseArgs<R, S> a) {
+  if constexpr (KIND >= 2) {
+    if (a.v.check_state && (!a.v.st->active || a.v.st->commit_pending)) return;
+  }
+  using Acc = typename Store<S>::Acc;
+  constexpr int VEC = Store<S>::VEC;
+  constexpr long STRIDE = (long)kBlock * VEC;
+  const int tid = threadIdx.x;
+  const long row0 = (long)blockIdx.x * ROWS;
+  const S *Ab = a.A + row0 * a.lda;
+  const R *cosv = a.v.cs_in;
+  const R *sinv = a.v.cs_in + a.v.ldv;
+
+  Acc accC[ROWS], accS[ROWS];
+#pragma unroll
+  for (int r = 0; r < ROWS; ++r) { accC[r] = 0; accS[r] = 0; }
+
+  auto load_row = [&](const S *p) -> u32x4 {
+    if constexpr (NT) return __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(p));
+    else return *reinterpret_cast<const u32x4 *>(p);
+  };
+
+  long c = (long)tid * VEC;
+  for (; c + STRIDE < a.lda; c += 2 * STRIDE) {
+    R cv0[VEC], sv0[VEC], cv1[VEC], sv1[VEC];
+    u32x4 raw0[ROWS], raw1[ROWS];
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) raw0[r] = load_row(Ab + r * a.lda + c);
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) raw1[r] = load_row(Ab + r * a.lda + c + STRIDE);
+    load_reals<R, VEC>(cosv + c, cv0);
+    load_reals<R, VEC>(sinv + c, sv0);
+    load_reals<R, VEC>(cosv + c + STRIDE, cv1);
+    load_reals<R, VEC>(sinv + c + STRIDE, sv1);
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+      Acc m[VEC];
+      unpack<S>(raw0[r], m);
+#pragma unroll
+      for (int q = 0; q < VEC; ++q) {
+        accC[r] = __builtin_fma(m[q], (Acc)cv0[q], accC[r]);
+        accS[r] = __builtin_fma(m[q], (Acc)sv0[q], accS[r]);
+      }
+      unpack<S>(raw1[r], m);
+#pragma unroll
+      for (int q = 0; q < VEC; ++q) {
+        accC[r] = __builtin_fma(m[q], (Acc)cv1[q], accC[r]);
+        accS[r] = __builtin_fma(m[q], (Acc)sv1[q], accS[r]);
+      }
+    }
+  }
+  if (c < a.lda) {
+    R cv0[VEC], sv0[VEC];
+    u32x4 raw0[ROWS];
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) raw0[r] = load_row(Ab + r * a.lda + c);
+    load_reals<R, VEC>(cosv + c, cv0);
+    load_reals<R, VEC>(sinv + c, sv0);
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+      Acc m[VEC];
+      unpack<S>(raw0[r], m);
+#pragma unroll
+      for (int q = 0; q < VEC; ++q) {
+        accC[r] = __builtin_fma(m[q], (Acc)cv0[q], accC[r]);
+        accS[r] = __builtin_fma(m[q], (Acc)sv0[q], accS[r]);
+      }
+    }
+  }
+
+  __shared__ Acc red[kBlock / kWave][2 * ROWS];
+  __shared__ double errsq[ROWS];
+  const int lane = tid & (kWave - 1), wave = tid / kWave;
+#pragma unroll
+  for (int r = 0; r < ROWS; ++r) {
+    accC[r] = wave_sum(accC[r]);
+    accS[r] = wave_sum(accS[r]);
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) { red[wave][r] = accC[r]; red[wave][ROWS + r] = accS[r]; }
+  }
+  __syncthreads();
+  if (tid < ROWS) {
+    const long i = row0 + tid;
+    double sq = 0.0;
+    if (i < a.v.N) {
+      Acc C = red[0][tid], Sv = red[0][ROWS + tid];
+#pragma unroll
+      for (int w = 1; w < kBlock / kWave; ++w) { C += red[w][tid]; Sv += red[w][ROWS + tid]; }
+      const R h = (KIND >= 2) ? a.v.st->h : (R)0;
+      sq = row_epilogue<R, KIND>(a.v, i, (R)C * a.mscale, (R)Sv * a.mscale, h);
+    }
+    if constexpr (KIND == 7) errsq[tid] = sq;
+  }
+  if constexpr (KIND == 7) {
+    __syncthreads();
+    if (tid == 0) {
+      double s = 0.0;
+#pragma unroll
+      for (int r = 0; r < ROWS; ++r) s += errsq[r];
+      a.v.err_part[blockIdx.x] = s;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------
+// Factored (-A) coupling: M = X diag(w) X^T (makecoupling's m1/m2 branch,
+// kuramoto.cu:62-108).  G_m = w_m sum_n X_nm (cos_n, sin_n); C_n =
+// sum_m X_nm G_m.  Three small launches per RHS; all sums in double.
+// ---------------------------------------------------------------------
+constexpr int kRankTile = 8;
+
+template <class R>
+__global__ __launch_bounds__(kBlock) void lowrank_partial_kernel(const R *X, int K, long N, const R *cs, long ldv,
+                                                                 double *part, const State<R> *st, int check_state) {
+  if (check_state && (!st->active || st->commit_pending)) return;
+  __shared__ double sh[kBlock];
+  const int m0 = blockIdx.y * kRankTile;
+  double ac[kRankTile], as[kRankTile];
+#pragma unroll
+  for (int q = 0; q < kRankTile; ++q) { ac[q] = 0; as[q] = 0; }
+  for (long n = (long)blockIdx.x * kBlock + threadIdx.x; n < N; n += (long)gridDim.x * kBlock) {
+    const double cv = cs[n], sv = cs[ldv + n];
+#pragma unroll
+    for (int q = 0; q < kRankTile; ++q) {
+      if (m0 + q < K) {
+        const double x = X[n * K + m0 + q];
+        ac[q] = __builtin_fma(x, cv, ac[q]);
+        as[q] = __builtin_fma(x, sv, as[q]);
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < kRankTile; ++q) {
+    const double tc = block_sum(ac[q], sh);
+    const double ts = block_sum(as[q], sh);
+    if (threadIdx.x == 0 && m0 + q < K) {
+      part[((size_t)blockIdx.x * K + m0 + q) * 2] = tc;
+      part[((size_t)blockIdx.x * K + m0 + q) * 2 + 1] = ts;
+    }
+  }
+}
+
+template <class R>
+__global__ __launch_bounds__(kBlock) void lowrank_reduce_kernel(const double *part, int nblk, int K, const R *w,
+                                                                double *G, const State<R> *st, int check_state) {
+  if (check_state && (!st->active || st->commit_pending)) return;
+  __shared__ double sh[kBlock];
+  for (int v = 0; v < 2 * K; ++v) {
+    double s = 0;
+    for (int b = threadIdx.x; b < nblk; b += kBlock) s += part[(size_t)b * 2 * K + v];
+    const double tot = block_sum(s, sh);
+    if (threadIdx.x == 0) G[v] = (double)w[v / 2] * tot;
+  }
+}
+
+template <class R, int KIND>
+__global__ __launch_bounds__(kBlock) void lowrank_expand_kernel(const R *X, int K, const double *G, Vecs<R> v) {
+  if constexpr (KIND >= 2) {
+    if (v.check_state && (!v.st->active || v.st->commit_pending)) return;
+  }
+  __shared__ double errsq[kBlock];
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  double sq = 0.0;
+  if (i < v.N) {
+    double C = 0, S = 0;
+    for (int m = 0; m < K; ++m) {
+      const double x = X[i * K + m];
+      C = __builtin_fma(x, G[2 * m], C);
+      S = __builtin_fma(x, G[2 * m + 1], S);
+    }
+    const R h = (KIND >= 2) ? v.st->h : (R)0;
+    sq = row_epilogue<R, KIND>(v, i, (R)C, (R)S, h);
+  }
+  if constexpr (KIND == 7) {
+    const double tot = block_sum(sq, errsq);
+    if (threadIdx.x == 0) v.err_part[blockIdx.x] = tot;
+  }
+}
+
+// ---------------------------------------------------------------------
+// Controller (dp45.cu:117-138): one workgroup.
+// ---------------------------------------------------------------------
+template <class R>
+__global__ __launch_bounds__(kBlock) void control_kernel(State<R> *st, const double *err_part, int nparts, long N,
+                                                         const R *t_eval, R *times) {
+  if (!st->active || st->commit_pending) return;
+  __shared__ double sh[kBlock];
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nparts; b += kBlock) s += err_part[b];
+  const double sum = block_sum(s, sh);
+  if (threadIdx.x != 0) return;
+  st->attempts += 1;
+  st->rhs_evals += 6;
+  const R t = st->t;
+  R h = st->h;
+  int accept;
+  R hnext = h;
+  if (st->fixed) {
+    accept = 1;
+  } else {
+    const R nrm = (R)sqrt(sum);                                // {S,D}nrm2
+    const R norm = (R)((double)nrm / sqrt((double)N));        // dp45.cu:119
+    R factor = (R)(0.9 * pow((double)norm, -0.2));            // dp45.cu:120
+    st->norm = (double)norm;
+    if (norm < (R)1) {
+      accept = 1;
+      if (factor > (R)10) factor = (R)10;
+      hnext = h * factor;
+    } else {
+      accept = 0;
+      if (factor < (R)1) {
+        if ((double)factor < 0.2) factor = (R)0.2;
+        hnext = h * factor;
+      }
+      if (isnan((double)norm)) {  // the reference would loop forever here
+        st->error = 1;
+        st->active = 0;
+      }
+    }
+  }
+  st->last_accept = accept;
+  if (!accept) {
+    st->rejected += 1;
+    st->h = hnext;
+    return;
+  }
+  const R tn = t + h;
+  st->t_last = t;
+  st->t = tn;
+  st->h = hnext;
+  st->accepted += 1;
+  if (st->times_count < st->times_cap) {
+    times[2 * st->times_count] = tn;
+    times[2 * st->times_count + 1] = hnext;
+    st->times_count += 1;
+  }
+  // output times crossed by this step (kuramoto.cu:205-212)
+  int j = st->eval_i;
+  while (j < st->n_eval && tn >= t_eval[j]) ++j;
+  st->pend_lo = st->eval_i;
+  st->pend_hi = j;
+  st->commit_pending = 1;
+}
+
+// ---------------------------------------------------------------------
+// Dense output + order-parameter partials + commit + next stage-2 vector.
+// ---------------------------------------------------------------------
+template <class R> struct EmitArgs {
+  State<R> *st;
+  long N, ldv;
+  R *y, *ylast, *ynew;
+  R *k[7];
+  R *cs_out;          // stage-2 (cos, sin) for the next attempt
+  const R *t_eval;
+  R *ring;            // ring_cap x ldv rows (or nullptr)
+  double *ord_part;   // ring_cap x gridDim.x x 2
+};
+
+template <class R> __device__ __forceinline__ void dense_coeffs(const R *kv, R Q[4]) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    R acc = kv[0] * (R)tab::P[0][q];
+#pragma unroll
+    for (int j = 1; j < 7; ++j) acc = __builtin_fma(kv[j], (R)tab::P[j][q], acc);
+    Q[q] = acc;
+  }
+}
+
+// ylast + (hh x) (Q0 + x (Q1 + x (Q2 + x Q3))),  x = (te - t_last) / hh
+template <class R> __device__ __forceinline__ R dense_value(const R Q[4], R yl, R hh, R x) {
+  R in = __builtin_fma(x, Q[3], Q[2]);
+  in = __builtin_fma(x, in, Q[1]);
+  in = __builtin_fma(x, in, Q[0]);
+  return __builtin_fma(hh * x, in, yl);
+}
+
+template <class R> __device__ __forceinline__ void stage2_vector(const EmitArgs<R> &a, long i, R y, R k1, R h) {
+  const R yt = __builtin_fma(k1 * (R)tab::A[0][0], h, y);
+  a.cs_out[i] = rcos<R>(yt);
+  a.cs_out[a.ldv + i] = rsin<R>(yt);
+}
+
+template <class R>
+__global__ __launch_bounds__(kBlock) void emit_commit_kernel(const EmitArgs<R> a) {
+  const State<R> *st = a.st;
+  if (!st->active) return;
+  __shared__ double sh[kBlock];
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  const bool valid = i < a.N;
+  const R h = st->h;
+  if (!st->commit_pending) {
+    if (valid) stage2_vector<R>(a, i, a.y[i], a.k[0][i], h);
+    return;
+  }
+  const int pend = st->pend_hi - st->pend_lo;
+  const int room = st->ring_cap - st->ring_count;
+  const int n_emit = pend < room ? pend : room;
+  if (n_emit > 0) {
+    const R t = st->t, tl = st->t_last;
+    const R hh = t - tl;
+    R Q[4] = {0, 0, 0, 0};
+    R yl = 0;
+    if (valid) {
+      R kv[7];
+#pragma unroll
+      for (int j = 0; j < 7; ++j) kv[j] = a.k[j][i];
+      dense_coeffs<R>(kv, Q);
+      yl = a.y[i];
+    }
+    for (int q = 0; q < n_emit; ++q) {
+      const int slot = st->ring_count + q;
+      const R te = a.t_eval[st->pend_lo + q];
+      const R x = (te - tl) / hh;
+      double cx = 0.0, sx = 0.0;
+      if (valid) {
+        const R val = dense_value<R>(Q, yl, hh, x);
+        if (a.ring) a.ring[(size_t)slot * a.ldv + i] = val;
+        cx = (double)rcos<R>(val);
+        sx = (double)rsin<R>(val);
+      }
+      const double bc = block_sum(cx, sh);
+      const double bs = block_sum(sx, sh);
+      if (threadIdx.x == 0) {
+        a.ord_part[((size_t)slot * gridDim.x + blockIdx.x) * 2] = bc;
+        a.ord_part[((size_t)slot * gridDim.x + blockIdx.x) * 2 + 1] = bs;
+      }
+    }
+  }
+  if (n_emit == pend && valid) {
+    // commit (dp45.cu:123-125, FSAL copy :160) and the next stage-2 vector
+    const R yo = a.y[i], yn = a.ynew[i], k7 = a.k[6][i];
+    a.ylast[i] = yo;
+    a.y[i] = yn;
+    a.k[0][i] = k7;
+    stage2_vector<R>(a, i, yn, k7, h);
+  }
+}
+
+// r = sqrt((X/N)^2 + (Y/N)^2) in the working precision (kuramoto.cu:220:
+// pow((X/N*X/N + Y/N*Y/N), 0.5) with X, Y the sums of cos and sin).
+template <class R> __device__ __forceinline__ R order_value(double X, double Y, long N) {
+  const R XR = (R)X, YR = (R)Y, NR = (R)N;
+  return (R)pow((double)(XR / NR * XR / NR + YR / NR * YR / NR), 0.5);
+}
+
+template <class R>
+__global__ __launch_bounds__(kBlock) void finalize_kernel(State<R> *st, const double *ord_part, int nblk, long N,
+                                                          R *r_ring) {
+  if (!st->active || !st->commit_pending) return;
+  __shared__ double sh[kBlock];
+  const int pend = st->pend_hi - st->pend_lo;
+  const int room = st->ring_cap - st->ring_count;
+  const int n_emit = pend < room ? pend : room;
+  const int base = st->ring_count;
+  for (int q = 0; q < n_emit; ++q) {
+    const int slot = base + q;
+    double sx = 0, sy = 0;
+    for (int b = threadIdx.x; b < nblk; b += kBlock) {
+      sx += ord_part[((size_t)slot * nblk + b) * 2];
+      sy += ord_part[((size_t)slot * nblk + b) * 2 + 1];
+    }
+    const double X = block_sum(sx, sh);
+    const double Y = block_sum(sy, sh);
+    if (threadIdx.x == 0) r_ring[slot] = order_value<R>(X, Y, N);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    st->ring_count = base + n_emit;
+    st->pend_lo += n_emit;
+    if (st->pend_lo == st->pend_hi) {
+      st->commit_pending = 0;
+      st->eval_i = st->pend_hi;
+      if (!(st->t < st->t1)) st->active = 0;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------
+// Small helpers: sin/cos of a phase vector, stand-alone order parameter,
+// interpolation over the last accepted step, adjacency generation and
+// conversion.
+// ---------------------------------------------------------------------
+template <class R>
+__global__ __launch_bounds__(kBlock) void sincos_kernel(const R *y, long N, long ldv, R *cs) {
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (i < N) {
+    cs[i] = rcos<R>(y[i]);
+    cs[ldv + i] = rsin<R>(y[i]);
+  }
+}
+
+template <class R>
+__global__ __launch_bounds__(kBlock) void order_partial_kernel(const R *y, long N, double *part) {
+  __shared__ double sh[kBlock];
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  double c = 0, s = 0;
+  if (i < N) { c = (double)rcos<R>(y[i]); s = (double)rsin<R>(y[i]); }
+  const double bc = block_sum(c, sh);
+  const double bs = block_sum(s, sh);
+  if (threadIdx.x == 0) { part[2 * blockIdx.x] = bc; part[2 * blockIdx.x + 1] = bs; }
+}
+
+template <class R>
+__global__ __launch_bounds__(kBlock) void order_final_kernel(const double *part, int nblk, long N, double *out) {
+  __shared__ double sh[kBlock];
+  double sx = 0, sy = 0;
+  for (int b = threadIdx.x; b < nblk; b += kBlock) { sx += part[2 * b]; sy += part[2 * b + 1]; }
+  const double X = block_sum(sx, sh);
+  const double Y = block_sum(sy, sh);
+  if (threadIdx.x == 0) out[0] = (double)order_value<R>(X, Y, N);
+}
+
+// dp45_eval (dp45.cu:143-146) over the last accepted step.
+template <class R>
+__global__ __launch_bounds__(kBlock) void interp_kernel(const State<R> *st, long N, const R *ylast,
+                                                        const R *k0, const R *k1, const R *k2, const R *k3,
+                                                        const R *k4, const R *k5, const R *k6, R t_eval, R *out) {
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= N) return;
+  const R tl = st->t_last;
+  const R hh = st->t - tl;
+  const R x = (t_eval - tl) / hh;
+  const R kv[7] = {k0[i], k1[i], k2[i], k3[i], k4[i], k5[i], k6[i]};
+  R Q[4];
+  dense_coeffs<R>(kv, Q);
+  out[i] = dense_value<R>(Q, ylast[i], hh, x);
+}
+
+__device__ __forceinline__ uint16_t to_bf16_rne(float f) {
+  uint32_t u = __builtin_bit_cast(uint32_t, f);
+  if ((u & 0x7FFFFFFFu) > 0x7F800000u) return (uint16_t)0x7FC0u;  // NaN
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+template <class R, class S> __device__ __forceinline__ S store_value(R v, R inv_scale);
+template <> __device__ __forceinline__ float store_value<float, float>(float v, float) { return v; }
+template <> __device__ __forceinline__ double store_value<double, double>(double v, double) { return v; }
+template <> __device__ __forceinline__ bf16_t store_value<float, bf16_t>(float v, float s) {
+  return bf16_t{to_bf16_rne(v * s)};
+}
+template <> __device__ __forceinline__ f16_t store_value<float, f16_t>(float v, float s) {
+  return f16_t{__builtin_bit_cast(uint16_t, (_Float16)(v * s))};
+}
+
+template <class R, class S> __device__ __forceinline__ R load_value(S v, R scale);
+template <> __device__ __forceinline__ float load_value<float, float>(float v, float) { return v; }
+template <> __device__ __forceinline__ double load_value<double, double>(double v, double) { return v; }
+template <> __device__ __forceinline__ float load_value<float, bf16_t>(bf16_t v, float s) {
+  return __builtin_bit_cast(float, (uint32_t)v.bits << 16) * s;
+}
+template <> __device__ __forceinline__ float load_value<float, f16_t>(f16_t v, float s) {
+  return (float)__builtin_bit_cast(_Float16, v.bits) * s;
+}
+
+// getadj (kuramoto.cu:145-161) from the stored factor: one workgroup per
+// row, M_ij = sum_m fma(w_m, X_im X_jm, acc) (exactly symmetric).
+template <class R, class S>
+__global__ __launch_bounds__(kBlock) void gen_adj_kernel(const R *X, const R *w, int K, long N, long row0, S *A,
+                                                         long lda, R inv_scale) {
+  const long i = row0 + blockIdx.x;
+  if (i >= N) return;
+  const R *Xi = X + i * K;
+  S *Ai = A + (size_t)blockIdx.x * lda;
+  for (long j = threadIdx.x; j < N; j += kBlock) {
+    const R *Xj = X + j * K;
+    R acc = 0;
+    for (int m = 0; m < K; ++m) acc = __builtin_fma(w[m], Xi[m] * Xj[m], acc);
+    Ai[j] = store_value<R, S>(acc, inv_scale);
+  }
+}
+
+// Host-supplied rows (real) -> stored rows.
+template <class R, class S>
+__global__ __launch_bounds__(kBlock) void convert_rows_kernel(const R *src, long ld_src, long ncols, long nrows, S *dst,
+                                                              long lda, R inv_scale) {
+  const long r = blockIdx.y;
+  if (r >= nrows) return;
+  for (long j = (long)blockIdx.x * kBlock + threadIdx.x; j < ncols; j += (long)gridDim.x * kBlock)
+    dst[(size_t)r * lda + j] = store_value<R, S>(src[(size_t)r * ld_src + j], inv_scale);
+}
+
+// Stored rows -> real rows (dequantized).
+template <class R, class S>
+__global__ __launch_bounds__(kBlock) void read_rows_kernel(const S *A, long lda, long ncols, long nrows, R scale,
+                                                           R *dst) {
+  const long r = blockIdx.y;
+  if (r >= nrows) return;
+  for (long j = (long)blockIdx.x * kBlock + threadIdx.x; j < ncols; j += (long)gridDim.x * kBlock)
+    dst[(size_t)r * ncols + j] = load_value<R, S>(A[(size_t)r * lda + j], scale);
+}
+
+}  // namespace kmt

@@ -1,0 +1,28 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs through the HIP library)")
+
+
+@pytest.fixture(scope="session")
+def orc():
+    from oracle import oracle_py
+
+    oracle_py.load()
+    return oracle_py
+
+
+@pytest.fixture(scope="session")
+def km():
+    import kuramoto_amd
+
+    kuramoto_amd.load()
+    return kuramoto_amd

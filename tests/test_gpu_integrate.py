@@ -1,0 +1,214 @@
+"""GPU parity of the fused DP45 integrator against the oracle and the
+reference's own K=0 fixtures, through the C ABI.
+
+Reference: dp45_step / dp45_run / dp45_eval (dp45.cu:87-164), step_eval's
+output grid and order parameter (kuramoto.cu:164-249), main's final-state
+interpolation (kuramoto.cu:630-647).
+
+Tolerances (north_star): phases at t=1 within 1e-10 (FP64) and 1e-5 (FP32,
+relative to max(1, |theta|)); r(t) within 1e-3; accepted-step counts within
+2%.  The K=0 controller KAT is bit-exact in t and step count.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "kat_k0.json")
+
+
+def _grid(t0, t1, dt, dtype):
+    dt = dtype(dt)
+    n0 = int(dtype(t0) / dt) + 1
+    n1 = int(dtype(t1) / dt) + 1
+    return (dt * np.arange(n0, n1)).astype(dtype)
+
+
+@pytest.mark.parametrize("name", ["5000", "5000_lorentz"])
+def test_k0_controller_kat(km, orc, name):
+    """data/5000{,_lorentz}/0/1fs.dat: N=5000, K=0, c=1.75, t=100, dt=0.01,
+    seed 1.  The GPU run must end at the fixture's t (bit-exact) with the
+    same number of accepted steps as the oracle, h within 4 ulp, phases at
+    the last accepted step within 8 ulp; and equal the oracle bit-for-bit."""
+    fx = json.load(open(GOLD))[name]
+    N = fx["N"]
+    print(f"case kat {name}", flush=True)
+    theta0, omega = km.draws(km.F32, N, fx["seed"], normal=fx["normal"], freq=1.0, init=0.0)
+    te = _grid(0.0, fx["t1"], fx["dt"], np.float32)
+    h0 = float(np.float32(fx["dt"]) / np.float32(100))
+    with km.Integrator(N, km.F32, atol=fx["atol"], c1=fx["c1"]) as it:
+        it.set_omega(omega)
+        it.set_factor(np.zeros((N, 0), np.float32), np.zeros(0, np.float32))
+        it.use_factored(True)
+        it.set_state(theta0, 0.0, h0)
+        res = it.run(te, fx["t1"], keep_rows=False, ring_rows=4096, chunk=64)
+        y, t, h = it.get_state()
+    st = res["status"]
+    assert np.float32(t).view(np.uint32) == fx["t_bits"]
+    assert abs(int(np.float32(h).view(np.int32)) - int(np.uint32(fx["h_bits"]).view(np.int32))) <= 4
+    for i, bits in fx["theta_samples"]:
+        assert abs(int(y[i:i + 1].view(np.int32)[0]) - int(np.uint32(bits).view(np.int32))) <= 8
+    ref = orc.integrate(orc.Adj(2, X=np.zeros((N, 0)), w=np.zeros(0)), omega, fx["c1"], theta0, 0.0, h0, fx["t1"],
+                        fx["atol"], t_eval=te, keep_rows=False, dtype=np.float32)
+    assert st.accepted == ref["accepted"] and st.rejected == ref["rejected"]
+    assert np.float32(h) == np.float32(ref["h"])
+    assert np.array_equal(y.view(np.uint32), ref["y"].view(np.uint32))
+    assert len(res["r"]) == len(te)
+    assert np.max(np.abs(res["r"] - ref["r"])) < 1e-6
+
+
+def _inputs(km, prec, N, K, seed, normal, init, gauss=True):
+    dtype = np.float32 if prec == km.F32 else np.float64
+    theta0, omega = km.draws(prec, N, seed, normal=normal, freq=1.0, init=init)
+    X = km.factor(N, K, seed, gaussian=gauss).astype(dtype)
+    w = km.factor_weights(prec, N, K, gaussian=gauss)
+    return dtype, theta0, omega, X, w
+
+
+@pytest.mark.parametrize("prec,atol,tol", [("f64", 1e-9, 1e-10), ("f32", 1e-6, 1e-5)])
+@pytest.mark.parametrize("N", [333, 2000])
+def test_dense_integration_matches_oracle(km, orc, prec, atol, tol, N):
+    p = km.F32 if prec == "f32" else km.F64
+    K, seed, c1, t1 = 5, 2, 3.0, 1.0
+    dtype, theta0, omega, X, w = _inputs(km, p, N, K, seed, normal=True, init=6.283185307)
+    te = _grid(0.0, t1, 0.1, dtype)
+    print(f"case dense_integration N={N} prec={prec}", flush=True)
+    with km.Integrator(N, p, atol=atol, c1=c1) as it:
+        it.set_omega(omega)
+        it.set_factor(X, w)
+        it.build_adjacency("native")
+        it.set_state(theta0, 0.0, 0.01)
+        g = it.run(te, t1, keep_rows=True, ring_rows=8)
+        fin = it.eval(float(te[-1]))
+    M = orc.adj_rows(X, w, 0, N)
+    ref = orc.integrate(orc.Adj(0, M=M), omega, c1, theta0, 0.0, 0.01, t1, atol, t_eval=te, dtype=dtype)
+    st = g["status"]
+    assert abs(st.accepted - ref["accepted"]) <= max(1, int(0.02 * ref["accepted"]))
+    assert g["rows"].shape == ref["rows"].shape
+    rel = np.abs(g["rows"].astype(np.float64) - ref["rows"]) / np.maximum(1.0, np.abs(ref["rows"]))
+    assert rel[-1].max() < tol  # phases at t = 1
+    assert np.max(np.abs(g["r"] - ref["r"])) < 1e-3
+    assert np.max(np.abs(fin - ref["final"]) / np.maximum(1.0, np.abs(ref["final"]))) < tol * 10
+    assert len(g["times"]) == st.accepted
+
+
+def test_ring_stall_matches_large_ring(km):
+    """A one-row output ring pauses the device at every output; the results
+    must equal a run with a large ring."""
+    N, K, seed, c1, t1 = 500, 5, 4, 3.0, 2.0
+    dtype, theta0, omega, X, w = _inputs(km, km.F64, N, K, seed, normal=False, init=6.283185307)
+    te = _grid(0.0, t1, 0.01, dtype)
+    outs = []
+    for ring in (1, 256):
+        with km.Integrator(N, km.F64, atol=1e-8, c1=c1) as it:
+            it.set_omega(omega)
+            it.set_factor(X, w)
+            it.build_adjacency()
+            it.set_state(theta0, 0.0, 0.0001)
+            outs.append(it.run(te, t1, keep_rows=True, ring_rows=ring, chunk=4))
+    a, b = outs
+    assert np.array_equal(a["rows"], b["rows"]) and np.array_equal(a["r"], b["r"])
+    assert np.array_equal(a["times"], b["times"])
+    assert len(a["r"]) == len(te)
+
+
+def test_fixed_step_matches_oracle(km, orc):
+    N, K, seed, c1 = 400, 5, 9, 3.0
+    dtype, theta0, omega, X, w = _inputs(km, km.F64, N, K, seed, normal=True, init=1.0)
+    te = _grid(0.0, 1.0, 0.25, dtype)
+    with km.Integrator(N, km.F64, atol=1e-6, c1=c1, fixed=True) as it:
+        it.set_omega(omega)
+        it.set_factor(X, w)
+        it.build_adjacency()
+        it.set_state(theta0, 0.0, 0.05)
+        g = it.run(te, 1.0, keep_rows=True)
+    M = orc.adj_rows(X, w, 0, N)
+    ref = orc.integrate(orc.Adj(0, M=M), omega, c1, theta0, 0.0, 0.05, 1.0, 1e-6, fixed=True, t_eval=te)
+    assert g["status"].accepted == ref["accepted"] == 20 and g["status"].rejected == 0
+    assert np.max(np.abs(g["rows"] - ref["rows"])) < 1e-11
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_factored_integration_matches_dense(km, orc, prec):
+    """-A (factored) and the dense matrix built from the same factor follow
+    the same trajectory (reference: kuramoto vs kuramoto -A)."""
+    p = km.F32 if prec == "f32" else km.F64
+    N, K, seed, c1, t1 = 1500, 5, 3, 3.0, 1.0
+    atol, tol = (1e-6, 1e-5) if prec == "f32" else (1e-9, 1e-10)
+    dtype, theta0, omega, X, w = _inputs(km, p, N, K, seed, normal=False, init=0.0)
+    te = _grid(0.0, t1, 0.1, dtype)
+    res = {}
+    for mode in ("dense", "factored"):
+        with km.Integrator(N, p, atol=atol, c1=c1) as it:
+            it.set_omega(omega)
+            it.set_factor(X, w)
+            if mode == "dense":
+                it.build_adjacency()
+            else:
+                it.use_factored(True)
+            it.set_state(theta0, 0.0, 0.01)
+            res[mode] = it.run(te, t1, keep_rows=True)
+    ref = orc.integrate(orc.Adj(2, X=X, w=w), omega, c1, theta0, 0.0, 0.01, t1, atol, t_eval=te, dtype=dtype)
+    for mode in res:
+        rel = np.abs(res[mode]["rows"][-1] - ref["rows"][-1]) / np.maximum(1.0, np.abs(ref["rows"][-1]))
+        assert rel.max() < tol, (mode, rel.max())
+        assert abs(res[mode]["status"].accepted - ref["accepted"]) <= max(1, int(0.02 * ref["accepted"]))
+
+
+def test_resume_is_bit_exact(km):
+    """Stopping after some steps and restarting from kmt_get_state (the
+    FILEBASEfs.dat restart of kuramoto.cu:471-494) continues bit-exactly:
+    the restart recomputes k1 = f(y), equal to the FSAL k7."""
+    N, K, seed, c1, t1 = 700, 5, 6, 3.0, 3.0
+    dtype, theta0, omega, X, w = _inputs(km, km.F64, N, K, seed, normal=True, init=6.283185307)
+    te = _grid(0.0, t1, 0.5, dtype)
+
+    def make():
+        it = km.Integrator(N, km.F64, atol=1e-9, c1=c1)
+        it.set_omega(omega)
+        it.set_factor(X, w)
+        it.build_adjacency()
+        return it
+
+    with make() as it:
+        it.set_state(theta0, 0.0, 0.01)
+        full = it.run(te, t1, keep_rows=False)
+        yf, tf, hf = it.get_state()
+    with make() as it:
+        it.set_state(theta0, 0.0, 0.01)
+        part = it.run(te, t1, keep_rows=False, chunk=5, max_attempts=15)
+        y1, t1_, h1 = it.get_state()
+    te2 = _grid(t1_, t1, 0.5, dtype)
+    with make() as it:
+        it.set_state(y1, t1_, h1)
+        rest = it.run(te2, t1, keep_rows=False)
+        y2, t2, h2 = it.get_state()
+    assert t2 == tf and h2 == hf
+    assert np.array_equal(y2, yf)
+    assert np.array_equal(np.concatenate([part["times"], rest["times"]]), full["times"])
+
+
+def test_step_and_eval_api(km, orc):
+    """kmt_step (dp45_step) and kmt_eval (dp45_eval) one attempt at a time."""
+    N, K, seed, c1 = 300, 5, 8, 3.0
+    dtype, theta0, omega, X, w = _inputs(km, km.F64, N, K, seed, normal=True, init=6.283185307)
+    with km.Integrator(N, km.F64, atol=1e-9, c1=c1) as it:
+        it.set_omega(omega)
+        it.set_factor(X, w)
+        it.build_adjacency()
+        it.set_state(theta0, 0.0, 0.01)
+        it.begin([], 10.0, keep_rows=False, ring_rows=1)
+        acc_n = 0
+        for _ in range(20):
+            acc, t, h = it.step()
+            acc_n += acc
+        y, t, h = it.get_state()
+        mid = it.eval((t + it.advance(0).t_last) / 2)
+    M = orc.adj_rows(X, w, 0, N)
+    ref = orc.integrate(orc.Adj(0, M=M), omega, c1, theta0, 0.0, 0.01, 10.0, 1e-9, max_steps=acc_n)
+    assert abs(ref["t"] - t) < 1e-12
+    assert np.max(np.abs(ref["y"] - y)) < 1e-10
+    assert np.all(np.isfinite(mid))

@@ -1,0 +1,159 @@
+"""GPU parity of the RHS kernels (fused gemv-pair stage kernel, factored -A
+path, adjacency generator) against the oracle, through the C ABI.
+
+Reference: dydt / makecoupling / tot_kuramoto / makey2 (kuramoto.cu:42-55,
+:104-126) and getadj (kuramoto.cu:145-161).
+
+Tolerances (stated per test): FP64 sums agree to 1e-12 of the row scale
+sum_j |M_ij| (different summation order only); FP32 sums accumulate in float
+on the GPU and in double in the oracle, so they agree to 2e-6 of the row
+scale; the adjacency generator is bit-exact (same fma chain on both sides).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [1, 2, 3, 63, 64, 65, 1000, 4097]
+
+
+def _case(N, dtype, seed=0, symmetric=True):
+    rng = np.random.default_rng(seed + N)
+    M = rng.standard_normal((N, N)) / max(N, 1)
+    if symmetric:
+        M = 0.5 * (M + M.T)
+    omega = rng.standard_normal(N)
+    theta = rng.uniform(-np.pi, np.pi, N) * 3
+    return M.astype(dtype), omega.astype(dtype), theta.astype(dtype)
+
+
+def _scale(M, omega, c1):
+    return np.abs(omega).astype(np.float64) + abs(c1) * np.abs(M.astype(np.float64)).sum(axis=1)
+
+
+@pytest.mark.parametrize("N", SIZES)
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_rhs_dense_matches_oracle(km, orc, N, prec):
+    dtype = np.float32 if prec == "f32" else np.float64
+    p = km.F32 if prec == "f32" else km.F64
+    M, omega, theta = _case(N, dtype)
+    c1 = 1.75
+    print(f"case rhs_dense N={N} prec={prec}", flush=True)
+    with km.Integrator(N, p, c1=c1) as it:
+        it.set_omega(omega)
+        it.set_adjacency(M)
+        f, cp = it.rhs(theta, couplings=True)
+    fo, cso = orc.rhs(orc.Adj(0, M=M), omega, c1, theta, dtype=dtype)
+    tol = (2e-6 if prec == "f32" else 1e-12) * _scale(M, omega, c1)
+    assert np.all(np.abs(f.astype(np.float64) - fo) <= tol + 1e-30)
+    rs = np.abs(M.astype(np.float64)).sum(axis=1)
+    tolc = (2e-6 if prec == "f32" else 1e-12) * np.repeat(rs, 2)
+    assert np.all(np.abs(cp.astype(np.float64) - cso) <= tolc + 1e-30)
+
+
+def test_rhs_zero_adjacency_is_omega(km):
+    """M = 0  =>  f = omega exactly (closed form)."""
+    N = 777
+    rng = np.random.default_rng(1)
+    omega = rng.standard_normal(N).astype(np.float32)
+    theta = rng.standard_normal(N).astype(np.float32)
+    with km.Integrator(N, km.F32, c1=3.0) as it:
+        it.set_omega(omega)
+        it.set_adjacency(np.zeros((N, N), np.float32))
+        f = it.rhs(theta)
+    assert np.array_equal(f, omega)
+
+
+def test_rhs_adler_pair(km):
+    """N=2, M12=M21=a: phi' = dw + 2 c a sin(phi) with phi = th1 - th2
+    (reference sign, kuramoto.cu:45)."""
+    a, c1 = 0.3, 2.0
+    M = np.array([[0.0, a], [a, 0.0]])
+    omega = np.array([0.7, -0.2])
+    for phi in np.linspace(-3, 3, 13):
+        theta = np.array([0.4 + phi, 0.4])
+        with km.Integrator(2, km.F64, c1=c1) as it:
+            it.set_omega(omega)
+            it.set_adjacency(M)
+            f = it.rhs(theta)
+        assert abs((f[0] - f[1]) - (omega[0] - omega[1] + 2 * c1 * a * np.sin(phi))) < 1e-14
+
+
+@pytest.mark.parametrize("gauss", [True, False])
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_build_adjacency_bit_exact(km, orc, gauss, prec):
+    N, K, seed = 3001, 5, 7
+    dtype = np.float32 if prec == "f32" else np.float64
+    p = km.F32 if prec == "f32" else km.F64
+    X = km.factor(N, K, seed, gaussian=gauss).astype(dtype)
+    w = km.factor_weights(p, N, K, gaussian=gauss)
+    print(f"case build_adjacency gauss={gauss} prec={prec}", flush=True)
+    with km.Integrator(N, p) as it:
+        it.set_factor(X, w)
+        it.build_adjacency("native")
+        rows = it.adjacency_rows(0, N)
+    ref = orc.adj_rows(X, w, 0, N)
+    assert np.array_equal(rows.view(np.uint8), ref.view(np.uint8))
+    assert np.array_equal(rows, rows.T)  # exactly symmetric
+
+
+@pytest.mark.parametrize("storage", ["bf16", "f16"])
+def test_reduced_storage_rhs(km, orc, storage):
+    """BF16/FP16 storage with FP32 accumulation equals the FP32 kernel applied
+    to the dequantized matrix (checked against the oracle on that matrix)."""
+    N, K, seed, c1 = 2049, 5, 3, 3.0
+    X = km.factor(N, K, seed, gaussian=True).astype(np.float32)
+    w = km.factor_weights(km.F32, N, K, gaussian=True)
+    rng = np.random.default_rng(5)
+    omega = rng.standard_normal(N).astype(np.float32)
+    theta = rng.uniform(-np.pi, np.pi, N).astype(np.float32)
+    print(f"case reduced storage={storage}", flush=True)
+    with km.Integrator(N, km.F32, c1=c1) as it:
+        it.set_omega(omega)
+        it.set_factor(X, w)
+        it.build_adjacency(storage)
+        Mq = it.adjacency_rows(0, N)
+        scale = it.adjacency_scale()
+        f = it.rhs(theta)
+    assert scale > 0 and np.log2(scale) == np.round(np.log2(scale))
+    fo, _ = orc.rhs(orc.Adj(0, M=Mq), omega, c1, theta, dtype=np.float32)
+    tol = 2e-6 * _scale(Mq, omega, c1)
+    assert np.all(np.abs(f.astype(np.float64) - fo) <= tol)
+    # quantization error against the exact matrix stays at the format's epsilon
+    M = orc.adj_rows(X, w, 0, N)
+    rel = np.abs(Mq - M).max() / np.abs(M).max()
+    assert rel < (2 ** -8 if storage == "bf16" else 2 ** -10)
+
+
+@pytest.mark.parametrize("K", [0, 1, 5, 9, 20])
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_factored_rhs_matches_oracle(km, orc, K, prec):
+    N = 5003
+    dtype = np.float32 if prec == "f32" else np.float64
+    p = km.F32 if prec == "f32" else km.F64
+    X = km.factor(N, K, 11, gaussian=False).astype(dtype)
+    w = km.factor_weights(p, N, K, gaussian=False)
+    rng = np.random.default_rng(K)
+    omega = rng.standard_normal(N).astype(dtype)
+    theta = rng.uniform(-np.pi, np.pi, N).astype(dtype)
+    print(f"case factored K={K} prec={prec}", flush=True)
+    with km.Integrator(N, p, c1=1.75) as it:
+        it.set_omega(omega)
+        it.set_factor(X, w)
+        it.use_factored(True)
+        f = it.rhs(theta)
+    fo, _ = orc.rhs(orc.Adj(2, X=X, w=w), omega, 1.75, theta, dtype=dtype)
+    tol = (1e-6 if prec == "f32" else 1e-13) * (np.abs(omega) + 1.75 * K)
+    assert np.all(np.abs(f.astype(np.float64) - fo) <= tol + 1e-12 * (prec == "f32"))
+
+
+def test_order_parameter(km):
+    N = 10001
+    rng = np.random.default_rng(2)
+    th = rng.uniform(-np.pi, np.pi, N)
+    with km.Integrator(N, km.F64) as it:
+        r = it.order(th)
+    exact = abs(np.mean(np.exp(1j * th)))
+    assert abs(r - exact) < 1e-12
+    with km.Integrator(N, km.F64) as it:
+        assert abs(it.order(np.full(N, 0.3)) - 1.0) < 1e-14
