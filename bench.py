@@ -109,7 +109,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="fp64-n50000", choices=sorted(CONFIGS))
-    ap.add_argument("--cpu-steps", type=int, default=2, help="accepted steps of the CPU baseline sample (0: skip)")
+    ap.add_argument("--cpu-steps", type=int, default=20, help="accepted steps of the CPU baseline sample (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (see DESIGN.md, "

@@ -80,6 +80,11 @@ template <> struct Store<bf16_t> { static constexpr int VEC = 8; using Acc = flo
 template <> struct Store<f16_t> { static constexpr int VEC = 8; using Acc = float; };
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+// Note: reinterpret a loaded 16-byte vector as a whole (u32x4 -> f32x4) or
+// copy an element into a scalar first.  hipcc 7.2 mis-compiles
+// __builtin_bit_cast(float, v[q]) applied directly to a vector element (it
+// reads lane 0 for every q).
 
 __device__ __forceinline__ float bf16_lo(uint32_t w) { return __builtin_bit_cast(float, w << 16); }
 __device__ __forceinline__ float bf16_hi(uint32_t w) { return __builtin_bit_cast(float, w & 0xFFFF0000u); }
@@ -89,20 +94,23 @@ __device__ __forceinline__ float f16_hi(uint32_t w) { return (float)__builtin_bi
 // 16 bytes of the adjacency -> VEC accumulator-typed values.
 template <class S> __device__ __forceinline__ void unpack(const u32x4 v, typename Store<S>::Acc *out);
 template <> __device__ __forceinline__ void unpack<float>(const u32x4 v, float *o) {
-  o[0] = __builtin_bit_cast(float, v[0]); o[1] = __builtin_bit_cast(float, v[1]);
-  o[2] = __builtin_bit_cast(float, v[2]); o[3] = __builtin_bit_cast(float, v[3]);
+  const f32x4 f = __builtin_bit_cast(f32x4, v);
+  o[0] = f.x; o[1] = f.y; o[2] = f.z; o[3] = f.w;
 }
 template <> __device__ __forceinline__ void unpack<double>(const u32x4 v, double *o) {
-  o[0] = __builtin_bit_cast(double, ((uint64_t)v[1] << 32) | v[0]);
-  o[1] = __builtin_bit_cast(double, ((uint64_t)v[3] << 32) | v[2]);
+  const uint32_t w0 = v.x, w1 = v.y, w2 = v.z, w3 = v.w;
+  o[0] = __builtin_bit_cast(double, ((uint64_t)w1 << 32) | w0);
+  o[1] = __builtin_bit_cast(double, ((uint64_t)w3 << 32) | w2);
 }
 template <> __device__ __forceinline__ void unpack<bf16_t>(const u32x4 v, float *o) {
+  const uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-  for (int q = 0; q < 4; ++q) { o[2 * q] = bf16_lo(v[q]); o[2 * q + 1] = bf16_hi(v[q]); }
+  for (int q = 0; q < 4; ++q) { o[2 * q] = bf16_lo(w[q]); o[2 * q + 1] = bf16_hi(w[q]); }
 }
 template <> __device__ __forceinline__ void unpack<f16_t>(const u32x4 v, float *o) {
+  const uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-  for (int q = 0; q < 4; ++q) { o[2 * q] = f16_lo(v[q]); o[2 * q + 1] = f16_hi(v[q]); }
+  for (int q = 0; q < 4; ++q) { o[2 * q] = f16_lo(w[q]); o[2 * q + 1] = f16_hi(w[q]); }
 }
 
 // VEC consecutive reals (16- or 32-byte aligned).
@@ -112,13 +120,7 @@ template <class R, int VEC> __device__ __forceinline__ void load_reals(const R *
 #pragma unroll
   for (int w = 0; w < W; ++w) {
     const u32x4 v = *reinterpret_cast<const u32x4 *>(reinterpret_cast<const char *>(p) + 16 * w);
-    if constexpr (sizeof(R) == 4) {
-#pragma unroll
-      for (int q = 0; q < 4; ++q) o[4 * w + q] = __builtin_bit_cast(float, v[q]);
-    } else {
-      o[2 * w] = __builtin_bit_cast(double, ((uint64_t)v[1] << 32) | v[0]);
-      o[2 * w + 1] = __builtin_bit_cast(double, ((uint64_t)v[3] << 32) | v[2]);
-    }
+    unpack<R>(v, o + (16 / sizeof(R)) * w);
   }
 }
 
@@ -131,6 +133,9 @@ template <class R> __device__ __forceinline__ R rsin(R x) {
 }
 template <class R> __device__ __forceinline__ R rabs(R x) {
   if constexpr (sizeof(R) == 4) return fabsf(x); else return fabs(x);
+}
+template <class R> __device__ __forceinline__ R rfma(R a, R b, R c) {
+  if constexpr (sizeof(R) == 4) return __builtin_fmaf(a, b, c); else return __builtin_fma(a, b, c);
 }
 template <class R> __device__ __forceinline__ R rmax(R a, R b) {
   if constexpr (sizeof(R) == 4) return fmaxf(a, b); else return fmax(a, b);
@@ -184,7 +189,7 @@ constexpr int KIND_COUPLING = 1;  // couplings only
 // kuramoto.cu:45: f = w + c1 (sin th C - cos th S), contracted as
 // fma(c1, fma(sin, C, -(cos S)), w).
 template <class R> __device__ __forceinline__ R rhs_value(R om, R c1, R cs, R sn, R C, R S) {
-  return __builtin_fma(c1, __builtin_fma(sn, C, -(cs * S)), om);
+  return rfma<R>(c1, rfma<R>(sn, C, -(cs * S)), om);
 }
 
 // Row epilogue for oscillator i.  Returns the squared scaled error of
@@ -211,9 +216,9 @@ __device__ __forceinline__ double row_epilogue(const Vecs<R> &a, long i, R C, R 
 #pragma unroll
       for (int j = 1; j < s; ++j) {
         const R kj = (j == s - 1) ? f : a.k[j][i];
-        acc = __builtin_fma(kj, (R)tab::A[s - 1][j], acc);
+        acc = rfma<R>(kj, (R)tab::A[s - 1][j], acc);
       }
-      const R yt = __builtin_fma(acc, h, a.y[i]);
+      const R yt = rfma<R>(acc, h, a.y[i]);
       if constexpr (s + 1 == 7) a.ynew[i] = yt;
       a.cs_out[i] = rcos<R>(yt);
       a.cs_out[a.ldv + i] = rsin<R>(yt);
@@ -224,11 +229,11 @@ __device__ __forceinline__ double row_epilogue(const Vecs<R> &a, long i, R C, R 
 #pragma unroll
       for (int j = 1; j < 7; ++j) {
         const R kj = (j == 6) ? f : a.k[j][i];
-        acc = __builtin_fma(kj, (R)tab::E[j], acc);
+        acc = rfma<R>(kj, (R)tab::E[j], acc);
       }
       const R st = a.st->rtl;
       const R num = h * acc;
-      const R den = __builtin_fma(st, rmax<R>(rabs<R>(a.y[i]), rabs<R>(a.ynew[i])), a.st->atl);
+      const R den = rfma<R>(st, rmax<R>(rabs<R>(a.y[i]), rabs<R>(a.ynew[i])), a.st->atl);
       const R e = num / den;
       return (double)e * (double)e;
     }
@@ -290,14 +295,14 @@ __global__ __launch_bounds__(kBlock) void stage_kernel(const DenseArgs<R, S> a) 
       unpack<S>(raw0[r], m);
 #pragma unroll
       for (int q = 0; q < VEC; ++q) {
-        accC[r] = __builtin_fma(m[q], (Acc)cv0[q], accC[r]);
-        accS[r] = __builtin_fma(m[q], (Acc)sv0[q], accS[r]);
+        accC[r] = rfma<Acc>(m[q], (Acc)cv0[q], accC[r]);
+        accS[r] = rfma<Acc>(m[q], (Acc)sv0[q], accS[r]);
       }
       unpack<S>(raw1[r], m);
 #pragma unroll
       for (int q = 0; q < VEC; ++q) {
-        accC[r] = __builtin_fma(m[q], (Acc)cv1[q], accC[r]);
-        accS[r] = __builtin_fma(m[q], (Acc)sv1[q], accS[r]);
+        accC[r] = rfma<Acc>(m[q], (Acc)cv1[q], accC[r]);
+        accS[r] = rfma<Acc>(m[q], (Acc)sv1[q], accS[r]);
       }
     }
   }
@@ -314,8 +319,8 @@ __global__ __launch_bounds__(kBlock) void stage_kernel(const DenseArgs<R, S> a) 
       unpack<S>(raw0[r], m);
 #pragma unroll
       for (int q = 0; q < VEC; ++q) {
-        accC[r] = __builtin_fma(m[q], (Acc)cv0[q], accC[r]);
-        accS[r] = __builtin_fma(m[q], (Acc)sv0[q], accS[r]);
+        accC[r] = rfma<Acc>(m[q], (Acc)cv0[q], accC[r]);
+        accS[r] = rfma<Acc>(m[q], (Acc)sv0[q], accS[r]);
       }
     }
   }
@@ -515,21 +520,21 @@ template <class R> __device__ __forceinline__ void dense_coeffs(const R *kv, R Q
   for (int q = 0; q < 4; ++q) {
     R acc = kv[0] * (R)tab::P[0][q];
 #pragma unroll
-    for (int j = 1; j < 7; ++j) acc = __builtin_fma(kv[j], (R)tab::P[j][q], acc);
+    for (int j = 1; j < 7; ++j) acc = rfma<R>(kv[j], (R)tab::P[j][q], acc);
     Q[q] = acc;
   }
 }
 
 // ylast + (hh x) (Q0 + x (Q1 + x (Q2 + x Q3))),  x = (te - t_last) / hh
 template <class R> __device__ __forceinline__ R dense_value(const R Q[4], R yl, R hh, R x) {
-  R in = __builtin_fma(x, Q[3], Q[2]);
-  in = __builtin_fma(x, in, Q[1]);
-  in = __builtin_fma(x, in, Q[0]);
-  return __builtin_fma(hh * x, in, yl);
+  R in = rfma<R>(x, Q[3], Q[2]);
+  in = rfma<R>(x, in, Q[1]);
+  in = rfma<R>(x, in, Q[0]);
+  return rfma<R>(hh * x, in, yl);
 }
 
 template <class R> __device__ __forceinline__ void stage2_vector(const EmitArgs<R> &a, long i, R y, R k1, R h) {
-  const R yt = __builtin_fma(k1 * (R)tab::A[0][0], h, y);
+  const R yt = rfma<R>(k1 * (R)tab::A[0][0], h, y);
   a.cs_out[i] = rcos<R>(yt);
   a.cs_out[a.ldv + i] = rsin<R>(yt);
 }
@@ -719,7 +724,7 @@ __global__ __launch_bounds__(kBlock) void gen_adj_kernel(const R *X, const R *w,
   for (long j = threadIdx.x; j < N; j += kBlock) {
     const R *Xj = X + j * K;
     R acc = 0;
-    for (int m = 0; m < K; ++m) acc = __builtin_fma(w[m], Xi[m] * Xj[m], acc);
+    for (int m = 0; m < K; ++m) acc = rfma<R>(w[m], Xi[m] * Xj[m], acc);
     Ai[j] = store_value<R, S>(acc, inv_scale);
   }
 }

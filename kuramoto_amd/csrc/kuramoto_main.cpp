@@ -22,7 +22,8 @@
 //                       prefactor) instead of the volcano matrix
 //   --adj-dtype=T       HBM storage of the dense adjacency: native (default),
 //                       bf16 or f16 (FP32 program only; FP32 accumulation)
-//   --max-steps=n       stop after n accepted steps
+//   --max-steps=n       stop after n accepted steps, leaving the files as a
+//                       process killed at that point would (no final block)
 //   --chunk=n           maximum step attempts per host synchronisation
 #include <getopt.h>
 #include <sys/time.h>
@@ -316,6 +317,7 @@ int main(int argc, char *argv[]) {
   std::fclose(out);
 
   CALL(kmt_set_state(g_ctx, yloc.data(), (double)t, (double)h));
+  const R t_start = t;
 
   // t = t0 outputs, kuramoto.cu:587-626
   if (!reloaded) {
@@ -351,9 +353,11 @@ int main(int argc, char *argv[]) {
   std::vector<R> ts(4096), hs(4096), rows(keep_rows ? (size_t)ring_rows * N : 0), rv(ring_rows), cpl(2 * N);
   long steps = 0;
   int chunk = 1;
-  bool started = false;
+  bool started = false, stopped = false;
   for (;;) {
-    CALL(kmt_advance(g_ctx, chunk, &st));
+    int n = chunk;
+    if (max_steps > 0) n = (int)std::max<long>(1, std::min<long>(n, max_steps - steps));
+    CALL(kmt_advance(g_ctx, n, &st));
     started = true;
     chunk = std::min(chunk * 2, chunk_max);
     // per accepted step: verbose line and times.dat (kuramoto.cu:169-188)
@@ -363,8 +367,8 @@ int main(int argc, char *argv[]) {
       ++steps;
       if (verbose) {
         const double el = now_s() - start;
-        const double frac = (double)((ts[q] - (R)0) / (t1 - (R)0));
-        const double eta = el / ((double)((ts[q] + hs[q]) / t1)) * (1 - frac);
+        const double frac = (double)((ts[q] - t_start) / (t1 - t_start));
+        const double eta = el / ((double)((ts[q] - t_start + hs[q]) / (t1 - t_start))) * (1 - frac);
         std::printf("%.3f\t%1.3e\t%1.3e\t%f\t%li\t\r", frac, el, eta, (double)hs[q], steps);
         std::fflush(stdout);
         FILE *fo = std::fopen(fname(".out").c_str(), "ab");
@@ -396,9 +400,16 @@ int main(int argc, char *argv[]) {
       write_fs(yloc, (R)tt, (R)hh, floc);
     }
     if (!st.active) break;
-    if (max_steps > 0 && steps >= max_steps && st.pending == 0) break;
+    if (max_steps > 0 && steps >= max_steps && st.pending == 0) {
+      stopped = true;
+      break;
+    }
   }
   const double loop_s = now_s() - loop0;
+  if (stopped) {  // leave the files as a process killed after this step would
+    kmt_destroy(g_ctx);
+    return 0;
+  }
 
   // final state with couplings, kuramoto.cu:630-647: interpolation over the
   // last step with k1 already replaced by k7, integrator t and h

@@ -12,7 +12,8 @@
  *                  M = sum_m (-1)^m u_m u_m^T / N with Rademacher u.
  *   --rebuild      oracle only: rebuild dense rows of M from the factor at
  *                  every RHS instead of storing N^2 values.
- *   --max-steps n  stop after n accepted steps (bounded baseline windows).
+ *   --max-steps n  stop after n accepted steps and leave the files as a
+ *                  process killed at that point would (no final block).
  * A summary line "oracle: ..." with loop time and osc*steps/s goes to stdout.
  */
 #include <getopt.h>
@@ -324,16 +325,29 @@ int main(int argc, char *argv[]) {
 
   /* dp45_run, dp45.cu:148-164 */
   double loop0 = now_s();
+  bool stopped = false;
   dp.f(dp.y.data(), dp.k[0].data());
   while (t < t1) {
     int ok = dp.step(&t, &h);
     if (ok) {
       step_eval(t, h, p);
       dp.k[0] = dp.k[6];
-      if (max_steps > 0 && p.steps >= max_steps) break;
+      if (max_steps > 0 && p.steps >= max_steps) {
+        stopped = true;
+        break;
+      }
     }
   }
   double loop_s = now_s() - loop0;
+  int threads = 1;
+#ifdef _OPENMP
+  threads = omp_get_max_threads();
+#endif
+  if (stopped) {  // leave the files as a process killed after this step would
+    std::printf("oracle: stopped accepted=%d rejected=%ld rhs=%ld loop_seconds=%.6f osc_steps_per_s=%.6e threads=%d t=%.9g h=%.9g\n",
+                p.steps, dp.rejected, dp.rhs_evals, loop_s, (double)N * p.steps / loop_s, threads, (double)t, (double)h);
+    return 0;
+  }
 
   /* final block, kuramoto.cu:630-647: interpolation uses k1 already replaced
    * by k7 and the integrator t */
@@ -355,10 +369,6 @@ int main(int argc, char *argv[]) {
   std::fprintf(out, "\nsteps: %i\n", p.steps);
   std::fprintf(out, "runtime: %f\n", rt);
   std::fclose(out);
-  int threads = 1;
-#ifdef _OPENMP
-  threads = omp_get_max_threads();
-#endif
   std::printf("oracle: accepted=%d rejected=%ld rhs=%ld loop_seconds=%.6f osc_steps_per_s=%.6e threads=%d t=%.9g h=%.9g\n",
               p.steps, dp.rejected, dp.rhs_evals, loop_s, (double)N * p.steps / loop_s, threads, (double)t, (double)h);
   return 0;

@@ -1,0 +1,148 @@
+"""The drop-in command line: `kuramoto` / `kuramoto_64` (product, GPU) against
+the oracle's restatement of the reference main() (CPU), same arguments, same
+FILEBASE files (README.md:42-58; kuramoto.cu:164-265, :395-678).
+
+File layouts checked (s = 4 or 8 bytes):
+  frequencies.dat N s | adj.dat N^2 s | order.dat (1 + n_eval) s |
+  thetas.dat (1 + n_eval) N s | couplings.dat (1 + n_eval) 2N s |
+  times.dat accepted s | fs.dat (N + 2 + 2N) s
+"""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORC = os.path.join(ROOT, "oracle", "_build")
+BIN = os.path.join(ROOT, "kuramoto_amd", "bin")
+
+
+def run(prog, args, base, timeout=300):
+    cmd = [prog] + args + [base]
+    out = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
+    assert out.returncode == 0, (cmd, out.stdout[-2000:], out.stderr[-2000:])
+    return out.stdout
+
+
+def load(base, suffix, dt):
+    return np.fromfile(base + suffix, dtype=dt)
+
+
+def check_layout(base, N, n_out, dt, dense=3, A=False):
+    s = np.dtype(dt).itemsize
+    assert os.path.getsize(base + "frequencies.dat") == N * s
+    assert os.path.getsize(base + "order.dat") == n_out * s
+    assert os.path.getsize(base + "fs.dat") == (3 * N + 2) * s
+    if dense >= 1:
+        assert os.path.getsize(base + "thetas.dat") == n_out * N * s
+        assert os.path.getsize(base + "times.dat") % s == 0
+    if dense >= 2:
+        assert os.path.getsize(base + "couplings.dat") == n_out * 2 * N * s
+    if dense >= 3 and not A:
+        assert os.path.getsize(base + "adj.dat") == N * N * s
+    header = open(base + ".out").readline().split()
+    # dmd.py:40 reads N K t1 dt c seed from the first line
+    assert len(header) == 6 and int(header[0]) == N
+
+
+ARGS = ["-N", "120", "-K", "5", "-c", "3", "-t", "2", "-d", "0.25", "-s", "3", "-I", "6.283185307"]
+N_OUT = 1 + 8
+
+
+@pytest.mark.parametrize("prog,dt", [("kuramoto_oracle", np.float32), ("kuramoto_oracle_64", np.float64)])
+def test_oracle_cli_layout(tmp_path, orc, prog, dt):
+    base = str(tmp_path / "run")
+    run(os.path.join(ORC, prog), ARGS + ["-n"], base)
+    check_layout(base, 120, N_OUT, dt)
+    M = load(base, "adj.dat", dt).reshape(120, 120)
+    assert np.array_equal(M, M.T)
+
+
+def test_oracle_cli_kill_resume(tmp_path, orc):
+    """Kill after 7 accepted steps (--max-steps leaves the files as a killed
+    process would), rerun with -R: order/thetas/times/couplings and the
+    final fs.dat equal an uninterrupted run byte for byte (kuramoto.cu:471-494)."""
+    prog = os.path.join(ORC, "kuramoto_oracle_64")
+    a = str(tmp_path / "a")
+    b = str(tmp_path / "b")
+    run(prog, ARGS, a)
+    run(prog, ARGS + ["--max-steps", "7"], b)
+    assert os.path.getsize(b + "fs.dat") == (3 * 120 + 2) * 8
+    run(prog, ARGS + ["-R"], b)
+    for suf in ("order.dat", "thetas.dat", "times.dat", "couplings.dat", "fs.dat", "adj.dat", "frequencies.dat"):
+        assert open(a + suf, "rb").read() == open(b + suf, "rb").read(), suf
+
+
+# ------------------------------------------------------------- GPU: product
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_cli_matches_oracle_cli(tmp_path, km, orc, prec):
+    dt = np.float32 if prec == "f32" else np.float64
+    prog = os.path.join(BIN, "kuramoto" if prec == "f32" else "kuramoto_64")
+    oprog = os.path.join(ORC, "kuramoto_oracle" if prec == "f32" else "kuramoto_oracle_64")
+    g, o = str(tmp_path / "g"), str(tmp_path / "o")
+    args = ARGS + ["-n", "-a", "1e-6" if prec == "f32" else "1e-9"]
+    print(f"case cli {prec}", flush=True)
+    run(prog, args, g)
+    run(oprog, args, o)
+    check_layout(g, 120, N_OUT, dt)
+    for suf in ("frequencies.dat", "adj.dat"):
+        assert open(g + suf, "rb").read() == open(o + suf, "rb").read(), suf
+    rg, ro = load(g, "order.dat", dt), load(o, "order.dat", dt)
+    # r(t0): GPU block reduction vs the oracle's sequential sum (last-bit)
+    assert abs(rg[0] - ro[0]) < (1e-14 if prec == "f64" else 1e-6)
+    assert np.max(np.abs(rg - ro)) < 1e-3
+    tg, to = load(g, "times.dat", dt), load(o, "times.dat", dt)
+    assert abs(len(tg) - len(to)) <= max(1, int(0.02 * len(to)))
+    thg = load(g, "thetas.dat", dt).reshape(N_OUT, 120)
+    tho = load(o, "thetas.dat", dt).reshape(N_OUT, 120)
+    assert np.array_equal(thg[0], tho[0])
+    tol = 1e-9 if prec == "f64" else 1e-3
+    assert np.max(np.abs(thg - tho) / np.maximum(1, np.abs(tho))) < tol
+    fg, fo = load(g, "fs.dat", dt), load(o, "fs.dat", dt)
+    assert np.max(np.abs(fg[:120] - fo[:120]) / np.maximum(1, np.abs(fo[:120]))) < tol
+    cg = load(g, "couplings.dat", dt).reshape(N_OUT, 240)
+    co = load(o, "couplings.dat", dt).reshape(N_OUT, 240)
+    assert np.max(np.abs(cg - co)) < (1e-9 if prec == "f64" else 1e-3)
+
+
+@pytest.mark.gpu
+def test_cli_kill_resume_bit_exact(tmp_path, km):
+    prog = os.path.join(BIN, "kuramoto_64")
+    a, b = str(tmp_path / "a"), str(tmp_path / "b")
+    print("case cli resume", flush=True)
+    run(prog, ARGS, a)
+    run(prog, ARGS + ["--max-steps", "7"], b)
+    run(prog, ARGS + ["-R"], b)
+    for suf in ("order.dat", "thetas.dat", "times.dat", "couplings.dat", "fs.dat"):
+        assert open(a + suf, "rb").read() == open(b + suf, "rb").read(), suf
+
+
+@pytest.mark.gpu
+def test_cli_lowrank_and_reload(tmp_path, km, orc):
+    """-A (factored coupling, no adj.dat) against the oracle's -A; then -R
+    with the oracle's frequency and adjacency files gives the oracle's
+    trajectory (identical inputs through -R, as the north_star asks)."""
+    g, o = str(tmp_path / "g"), str(tmp_path / "o")
+    args = ARGS + ["-A", "-a", "1e-9", "-D", "2"]
+    print("case cli -A", flush=True)
+    run(os.path.join(BIN, "kuramoto_64"), args, g)
+    run(os.path.join(ORC, "kuramoto_oracle_64"), args, o)
+    assert not os.path.exists(g + "adj.dat")
+    check_layout(g, 120, N_OUT, np.float64, dense=2, A=True)
+    assert np.max(np.abs(load(g, "order.dat", np.float64) - load(o, "order.dat", np.float64))) < 1e-9
+    # -R: copy the oracle's inputs (adjacency, frequencies, a restart file)
+    r = str(tmp_path / "r")
+    o2 = str(tmp_path / "o2")
+    run(os.path.join(ORC, "kuramoto_oracle_64"), ARGS + ["-a", "1e-9", "-s", "8"], o2)
+    for suf in ("adj.dat", "frequencies.dat"):
+        shutil.copy(o2 + suf, r + suf)
+    th0 = load(o2, "thetas.dat", np.float64)[:120]
+    np.concatenate([th0, [0.0, 0.02]]).tofile(r + "fs.dat")
+    run(os.path.join(BIN, "kuramoto_64"), ARGS + ["-a", "1e-9", "-R", "-s", "99"], r)
+    ro = load(o2, "order.dat", np.float64)[1:]
+    rr = load(r, "order.dat", np.float64)
+    assert len(rr) == len(ro) and np.max(np.abs(rr - ro)) < 1e-3
+    assert "Using adjacency from file" in open(r + ".out").read()
