@@ -46,6 +46,19 @@ CONFIGS = {
 HBM_PEAK = 8.0e12
 
 
+def profiled_traffic(config):
+    """HBM bytes per stage launch measured by the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE runs recorded in profiles/stage_kernel_<config>.txt (None if
+    that workload has not been profiled)."""
+    path = os.path.join(ROOT, "profiles", "stage_kernel_" + config.replace("-", "_") + ".txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1])
+    return None
+
+
 def gpu_run(cfg, steps, warmup):
     import kuramoto_amd as km
 
@@ -73,8 +86,9 @@ def gpu_run(cfg, steps, warmup):
     region_ms, stage_ms, st = it.time_attempts(steps)
     lda, eb, rows = it.layout()
     it.close()
+    copy_gbps, read_gbps = km.bandwidth_probe(8 << 30, 5)
     return dict(dtype=dtype, N=N, region_ms=region_ms, stage_ms=stage_ms, st0=st0, st=st, lda=lda, elem_bytes=eb,
-                gen_s=gen_s)
+                gen_s=gen_s, rows=abs(rows), nt=rows > 0, copy_gbps=copy_gbps, read_gbps=read_gbps)
 
 
 def cpu_baseline(cfg, max_steps):
@@ -142,9 +156,12 @@ def main():
                    "rejected_in_region": r["st"].rejected - r["st0"].rejected,
                    "t_after": r["st"].t, "adjacency_build_s": round(r["gen_s"], 3)},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "frac": achieved / HBM_PEAK, "traffic": profiled_traffic(args.config),
                      "kernel": "stage_kernel (fused gemv-pair + DP45 stage epilogue)",
-                     "bytes_per_launch": alg_bytes, "avg_launch_ms": r["stage_ms"]},
+                     "bytes_per_launch": alg_bytes, "avg_launch_ms": r["stage_ms"],
+                     "variant": {"rows_per_workgroup": r["rows"], "nontemporal_loads": r["nt"]},
+                     "measured_read_stream_gbps": r["read_gbps"], "measured_copy_gbps": r["copy_gbps"],
+                     "frac_of_measured_read_stream": achieved / 1e9 / r["read_gbps"]},
     }
     if args.cpu_steps > 0:
         try:

@@ -160,9 +160,25 @@ int kmt_time_stage_kernel(kmt_ctx *ctx, int reps, double *avg_ms);
  * kernel launches inside it (events around each stage launch). */
 int kmt_time_attempts(kmt_ctx *ctx, int attempts, double *region_ms, double *stage_avg_ms, kmt_status *st);
 
+/* Measured bandwidth ceiling of the device: a plain 16-byte copy kernel
+ * (reads + writes `bytes` each way) and a read-only stream of `bytes`,
+ * `reps` launches each, timed with HIP events.  GB/s (1e9 B/s).  No
+ * reference counterpart: the denominator of the roofline next to the
+ * nominal peak, and the calibration kernel of the HBM byte counters. */
+int kmt_bandwidth_probe(int device, size_t bytes, int reps, double *copy_gbps, double *read_gbps);
+
+/* Autotune the dense stage kernel on this context's adjacency: rows per
+ * workgroup {4, 8, 16} x non-temporal loads {off, on}, `reps` timed launches
+ * each; keeps the fastest.  variant_ms (6 values, may be NULL) receives the
+ * average ms of each variant in the order (4,off) (4,on) (8,off) (8,on)
+ * (16,off) (16,on).  kmt_begin runs it once by itself for N >= 4096.  The
+ * variants differ in speed only: every row is summed in the same order.  No
+ * reference counterpart (cuBLAS chooses its own gemv kernel). */
+int kmt_autotune(kmt_ctx *ctx, int reps, double *variant_ms);
+
 /* Sizes the library chose (for roofline bookkeeping): padded row stride of
  * the adjacency in elements, bytes per stored element, rows per workgroup
- * of the stage kernel. */
+ * of the stage kernel (negative when the non-temporal load variant is off). */
 int kmt_layout(kmt_ctx *ctx, long *lda, int *elem_bytes, int *rows_per_block);
 
 #ifdef __cplusplus

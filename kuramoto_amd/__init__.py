@@ -89,6 +89,8 @@ _SIGNATURES = {
     "kmt_time_stage_kernel": ([_P, _I, _P], _I),
     "kmt_time_attempts": ([_P, _I, _P, _P, _P], _I),
     "kmt_layout": ([_P, _P, _P, _P], _I),
+    "kmt_autotune": ([_P, _I, _P], _I),
+    "kmt_bandwidth_probe": ([_I, ctypes.c_size_t, _I, _P, _P], _I),
 }
 
 
@@ -125,6 +127,13 @@ def device_count() -> int:
     n = ctypes.c_int(0)
     _check(load().kmt_device_count(ctypes.byref(n)), "kmt_device_count")
     return n.value
+
+
+def bandwidth_probe(nbytes=8 << 30, reps=5, device=0):
+    """(copy GB/s, read GB/s) of plain 16-byte streaming kernels."""
+    c, r = ctypes.c_double(), ctypes.c_double()
+    _check(load().kmt_bandwidth_probe(device, nbytes, reps, ctypes.byref(c), ctypes.byref(r)), "kmt_bandwidth_probe")
+    return c.value, r.value
 
 
 def draws(precision, N, seed, normal=False, freq=1.0, init=0.0, ics=True, freqs=True):
@@ -300,6 +309,11 @@ class Integrator:
         _check(self.lib.kmt_time_attempts(self.ctx, attempts, ctypes.byref(region), ctypes.byref(stage),
                                           ctypes.byref(st)), "kmt_time_attempts")
         return region.value, stage.value, st
+
+    def autotune(self, reps=3):
+        ms = np.zeros(6)
+        _check(self.lib.kmt_autotune(self.ctx, reps, _ptr(ms)), "kmt_autotune")
+        return ms
 
     def layout(self):
         lda, eb, rows = ctypes.c_long(), ctypes.c_int(), ctypes.c_int()

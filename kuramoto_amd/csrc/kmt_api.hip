@@ -50,7 +50,6 @@ static int fail(int code, const char *fmt, ...) {
 static inline long round_up(long x, long m) { return (x + m - 1) / m * m; }
 
 // Rows per workgroup of the stage kernel (per storage width).
-template <class S> struct RowsFor { static constexpr int value = 8; };
 
 // ---------------------------------------------------------------------
 // Engine
@@ -76,6 +75,7 @@ struct EngineBase {
   virtual int time_stage(int, double *) = 0;
   virtual int time_attempts(int, double *, double *, kmt_status *) = 0;
   virtual int layout(long *, int *, int *) = 0;
+  virtual int tune(int, double *) = 0;
   double mscale_ = 1.0;
 };
 
@@ -117,6 +117,8 @@ template <class R> struct Engine : EngineBase {
   int ring_cap = 0, keep_rows = 0, times_cap = 4096, nblk_emit = 0;
   hipGraphExec_t g1 = nullptr, g8 = nullptr;
   bool graphs_valid = false;
+  int tune_rows = 8, tune_nt = 1, tuned = 0;
+  double tune_ms[6] = {0, 0, 0, 0, 0, 0};
   R t0_ = 0, h0_ = 0;
 
   ~Engine() override { release(); }
@@ -193,7 +195,12 @@ template <class R> struct Engine : EngineBase {
   int layout(long *plda, int *eb, int *rows) override {
     if (plda) *plda = lda;
     if (eb) *eb = storage == KMT_ADJ_BF16 || storage == KMT_ADJ_F16 ? 2 : (int)sizeof(R);
-    if (rows) *rows = 8;
+    if (rows) *rows = tune_nt ? tune_rows : -tune_rows;
+    return KMT_OK;
+  }
+  int tune(int reps, double *ms6) override {
+    KMT_TRY(autotune(reps));
+    if (ms6) for (int q = 0; q < 6; ++q) ms6[q] = tune_ms[q];
     return KMT_OK;
   }
 
@@ -266,6 +273,7 @@ template <class R> struct Engine : EngineBase {
     }
     A_bytes = bytes;
     storage = stg;
+    tuned = 0;
     HIP_TRY(hipMemsetAsync(A, 0, bytes, stream));
     return KMT_OK;
   }
@@ -444,15 +452,25 @@ template <class R> struct Engine : EngineBase {
 
   int coupling_ready() const { return factored ? (X != nullptr) : (A != nullptr); }
 
+  template <class S, int KIND, int ROWS, bool NT> void launch_dense_variant(const DenseArgs<R, S> &a) {
+    const unsigned grid = (unsigned)((N + ROWS - 1) / ROWS);
+    stage_kernel<R, S, ROWS, KIND, NT><<<grid, kBlock, 0, stream>>>(a);
+  }
+
   template <class S, int KIND> void launch_dense_kind(const Vecs<R> &v) {
-    constexpr int ROWS = RowsFor<S>::value;
     DenseArgs<R, S> a;
     a.A = (const S *)A;
     a.lda = lda;
     a.mscale = (R)mscale_;
     a.v = v;
-    const unsigned grid = (unsigned)((N + ROWS - 1) / ROWS);
-    stage_kernel<R, S, ROWS, KIND, true><<<grid, kBlock, 0, stream>>>(a);
+    switch (tune_rows * 2 + (tune_nt ? 1 : 0)) {
+      case 8: launch_dense_variant<S, KIND, 4, false>(a); break;
+      case 9: launch_dense_variant<S, KIND, 4, true>(a); break;
+      case 16: launch_dense_variant<S, KIND, 8, false>(a); break;
+      case 32: launch_dense_variant<S, KIND, 16, false>(a); break;
+      case 33: launch_dense_variant<S, KIND, 16, true>(a); break;
+      default: launch_dense_variant<S, KIND, 8, true>(a); break;
+    }
   }
 
   template <int KIND> void launch_dense(const Vecs<R> &v) {
@@ -493,7 +511,48 @@ template <class R> struct Engine : EngineBase {
 
   int n_parts_stage7() const {
     if (factored) return (int)((N + kBlock - 1) / kBlock);
-    return (int)((N + 7) / 8);
+    return (int)((N + tune_rows - 1) / tune_rows);
+  }
+
+  // Autotuner of the dense stage kernel: rows per workgroup {4, 8, 16} x
+  // non-temporal adjacency loads {off, on}, each timed on this context's own
+  // matrix (stage 3 on scratch buffers).  Every variant sums each row in the
+  // same order, so the choice changes speed only, never results.
+  int autotune(int reps) {
+    if (factored || !A) return KMT_OK;
+    if (!k_scr) {
+      HIP_TRY(hipMalloc((void **)&k_scr, sizeof(R) * 7 * ldv));
+      HIP_TRY(hipMemsetAsync(k_scr, 0, sizeof(R) * 7 * ldv, stream));
+    }
+    Vecs<R> v = vecs(cs[0], cs_scr, 0);
+    for (int j = 0; j < 7; ++j) v.k[j] = k_scr + j * ldv;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    const int rows_opts[3] = {4, 8, 16};
+    double best = 1e30;
+    int best_rows = 8, best_nt = 1;
+    for (int ri = 0; ri < 3; ++ri)
+      for (int nt = 0; nt < 2; ++nt) {
+        tune_rows = rows_opts[ri];
+        tune_nt = nt;
+        launch_by_kind(3, v);
+        hipEventRecord(e0, stream);
+        for (int r = 0; r < reps; ++r) launch_by_kind(3, v);
+        hipEventRecord(e1, stream);
+        HIP_TRY(hipEventSynchronize(e1));
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        tune_ms[ri * 2 + nt] = ms / reps;
+        if (ms < best) { best = ms; best_rows = tune_rows; best_nt = nt; }
+      }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    tune_rows = best_rows;
+    tune_nt = best_nt;
+    tuned = 1;
+    drop_graphs();
+    return KMT_OK;
   }
 
   EmitArgs<R> emit_args() {
@@ -619,6 +678,7 @@ template <class R> struct Engine : EngineBase {
     shadow.times_cap = times_cap;
     shadow.rhs_evals = 1;
     KMT_TRY(push_state());
+    if (!tuned && !factored && N >= 4096) KMT_TRY(autotune(N >= 20000 ? 2 : 5));
     // k1 = f(y) (dp45.cu:151), then the first stage-2 vector
     sincos_kernel<R><<<(unsigned)((N + kBlock - 1) / kBlock), kBlock, 0, stream>>>(y, N, ldv, cs[1]);
     Vecs<R> v = vecs(cs[1], nullptr, 0);
@@ -1024,6 +1084,62 @@ int kmt_time_attempts(kmt_ctx *ctx, int attempts, double *region_ms, double *sta
 int kmt_layout(kmt_ctx *ctx, long *lda, int *elem_bytes, int *rows_per_block) {
   CTX_CHECK;
   return ctx->eng->layout(lda, elem_bytes, rows_per_block);
+}
+
+int kmt_autotune(kmt_ctx *ctx, int reps, double *variant_ms) {
+  CTX_CHECK;
+  if (reps < 1) return fail(KMT_ERR_ARG, "reps < 1");
+  return ctx->eng->tune(reps, variant_ms);
+}
+
+int kmt_bandwidth_probe(int device, size_t bytes, int reps, double *copy_gbps, double *read_gbps) {
+  if (bytes < (1u << 20) || reps < 1) return fail(KMT_ERR_ARG, "probe needs >= 1 MiB and reps >= 1");
+  HIP_TRY(hipSetDevice(device));
+  const size_t n16 = bytes / 16;
+  void *a = nullptr, *b = nullptr;
+  unsigned int *sink = nullptr;
+  hipStream_t s = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  auto cleanup = [&]() {
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (a) hipFree(a);
+    if (b) hipFree(b);
+    if (sink) hipFree(sink);
+    if (s) hipStreamDestroy(s);
+  };
+  hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc(&a, n16 * 16);
+  if (e == hipSuccess) e = hipMalloc(&b, n16 * 16);
+  if (e == hipSuccess) e = hipMalloc((void **)&sink, 64);
+  if (e == hipSuccess) e = hipMemsetAsync(a, 1, n16 * 16, s);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e != hipSuccess) {
+    cleanup();
+    return fail(e == hipErrorOutOfMemory ? KMT_ERR_MEMORY : KMT_ERR_HIP, "probe setup: %s", hipGetErrorString(e));
+  }
+  int cus = 256;
+  hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+  const unsigned grid = (unsigned)std::min<size_t>((n16 + kBlock - 1) / kBlock, (size_t)cus * 8);
+  float ms_copy = 0, ms_read = 0;
+  copy_probe_kernel<<<grid, kBlock, 0, s>>>((const u32x4 *)a, (u32x4 *)b, n16);
+  hipEventRecord(e0, s);
+  for (int r = 0; r < reps; ++r) copy_probe_kernel<<<grid, kBlock, 0, s>>>((const u32x4 *)a, (u32x4 *)b, n16);
+  hipEventRecord(e1, s);
+  e = hipEventSynchronize(e1);
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms_copy, e0, e1);
+  read_probe_kernel<<<grid, kBlock, 0, s>>>((const u32x4 *)a, n16, sink);
+  hipEventRecord(e0, s);
+  for (int r = 0; r < reps; ++r) read_probe_kernel<<<grid, kBlock, 0, s>>>((const u32x4 *)a, n16, sink);
+  hipEventRecord(e1, s);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms_read, e0, e1);
+  cleanup();
+  if (e != hipSuccess) return fail(KMT_ERR_HIP, "probe: %s", hipGetErrorString(e));
+  if (copy_gbps) *copy_gbps = 2.0 * n16 * 16 * reps / (ms_copy * 1e-3) / 1e9;
+  if (read_gbps) *read_gbps = 1.0 * n16 * 16 * reps / (ms_read * 1e-3) / 1e9;
+  return KMT_OK;
 }
 
 }  // extern "C"
