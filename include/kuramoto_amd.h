@@ -181,6 +181,99 @@ int kmt_autotune(kmt_ctx *ctx, int reps, double *variant_ms);
  * of the stage kernel (negative when the non-temporal load variant is off). */
 int kmt_layout(kmt_ctx *ctx, long *lda, int *elem_bytes, int *rows_per_block);
 
+/* ---- batched integration (north_star extension; no reference function:
+ * the reference runs one instance per process, SLURM arrays of
+ * kuramoto calls, job1.sh:13-25) ------------------------------------------
+ * B independent FP32 runs advanced in lockstep stages on one GPU, each with
+ * its own step size, accept/reject, output cursor and order-parameter ring.
+ * shared = 1: all instances have the same N and share one adjacency stored
+ * in BF16/FP16, coupled as one MFMA product A.[C S] per stage (B <= 64).
+ * shared = 0: instance b has its own N_b and its own adjacency (any
+ * storage), coupled by a grouped gemv-pair launch. */
+typedef struct kmt_batch kmt_batch;
+typedef struct kmt_batch_status {
+  long accepted;          /* summed over instances */
+  long rejected;
+  long attempts;          /* max over instances */
+  long rhs_evals;         /* summed */
+  int active;             /* instances still running */
+  int errors;             /* instances stopped by a non-finite error norm */
+  int rows_ready;         /* max order-parameter values waiting per instance */
+  int pad_;
+  double t_min, t_max;    /* instance times */
+  double osc_steps;       /* sum_b N_b * accepted_b */
+  long proposer_accepts;  /* attempts whose h came from the learned proposer */
+  long proposer_rejects;
+} kmt_batch_status;
+
+int kmt_batch_create(kmt_batch **batch, int device, int B, const long *N, double atol, double rtol, int shared,
+                     int storage, int fixed_step);
+int kmt_batch_destroy(kmt_batch *batch);
+/* frequencies, phases, coupling c1, start time and first step of instance b */
+int kmt_batch_set_instance(kmt_batch *batch, int b, const float *omega, const float *theta0, double c1, double t0,
+                           double h0);
+/* dense adjacency of instance b (b = -1: the shared one) built on the device
+ * from a factor X (N x K, FP32) and weights w: M = X diag(w) X^T */
+int kmt_batch_set_factor(kmt_batch *batch, int b, const float *X, const float *w, int K);
+/* dequantized rows [i0, i1) of instance b's adjacency (b = -1: shared) */
+int kmt_batch_get_adjacency_rows(kmt_batch *batch, int b, long i0, long i1, float *out);
+/* shared output grid and end time for all instances; ring_rows = r values
+ * per instance kept on the device between reads */
+int kmt_batch_begin(kmt_batch *batch, const double *t_eval, int n_eval, double t1, int ring_rows);
+int kmt_batch_advance(kmt_batch *batch, int max_attempts, kmt_batch_status *st);
+/* drain instance b's order parameters r(t_eval[j]) in output order */
+int kmt_batch_read_order(kmt_batch *batch, int b, int max_n, float *r, int *n);
+/* phases at the last accepted step; state8 = {t, h, accepted, rejected,
+ * active, error, proposer accepts, proposer rejects} (either may be NULL) */
+int kmt_batch_get_instance(kmt_batch *batch, int b, float *theta, double *state8);
+/* tile autotuner of the shared MFMA product: n-tiles per wave {1, 2, 4} x
+ * split-K {1, 2, 4, 8, 16}, `reps` launches each; keeps the fastest.
+ * kmt_batch_tune_table returns the 15 average times (ms), n-tile major. */
+int kmt_batch_autotune(kmt_batch *batch, int reps, int *n_tiles, int *split_k, double *best_ms);
+int kmt_batch_tune_table(kmt_batch *batch, double *ms15);
+/* HIP-event timing of `attempts` attempts; coupling_ms = average of the
+ * coupling launches (MFMA product or grouped gemv) inside the region */
+int kmt_batch_time_attempts(kmt_batch *batch, int attempts, double *region_ms, double *coupling_ms,
+                            kmt_batch_status *st);
+/* shared mode: 1 (default) = cos/sin as hi + lo 16-bit parts (two MFMAs),
+ * 0 = hi part only */
+int kmt_batch_set_vector_split(kmt_batch *batch, int hi_lo);
+/* learned step-size proposer (MLP 8-64-64-1, FP16 weights, FP32
+ * accumulation): after an accepted step the next h is the standard
+ * controller's proposal times exp(mlp(features)); the embedded error still
+ * accepts or rejects every step and a rejection falls back to the standard
+ * controller.  weights: kmt_proposer_param_count() FP16 values (NULL keeps
+ * the loaded ones). */
+int kmt_batch_set_proposer(kmt_batch *batch, const uint16_t *weights_f16, int n_params, int enabled);
+int kmt_proposer_param_count(void);
+/* attempt log for training the proposer: capacity records of 14 floats
+ * {8 features, h used, h proposed, error norm, accepted, instance,
+ *  1 if h came from the proposal made after an accepted step} */
+int kmt_batch_set_attempt_log(kmt_batch *batch, int capacity);
+int kmt_batch_read_attempt_log(kmt_batch *batch, int max_n, float *records, int *n);
+
+/* ---- training the learned step-size proposer on the GPU ----------------
+ * MLP 8 -> 64 -> 64 -> 1 (tanh, tanh, linear), parameters packed as
+ * W1[64][8], b1[64], W2[64][64], b2[64], W3[64], b3 (kmt_proposer_param_count()
+ * values).  FP16 copy for inference (the one kmt_batch_set_proposer takes),
+ * FP32 master weights for Adam; FP32 accumulation.  Loss: mean squared error. */
+typedef struct kmt_mlp kmt_mlp;
+/* w_init: FP32 parameters, or NULL for Glorot-uniform hidden layers from
+ * `seed` with a zero output layer (the untrained proposer returns the
+ * standard controller's step). */
+int kmt_mlp_create(kmt_mlp **mlp, int device, const float *w_init, uint32_t seed);
+int kmt_mlp_destroy(kmt_mlp *mlp);
+int kmt_mlp_get_weights(kmt_mlp *mlp, float *w32, uint16_t *w16);
+/* outputs for n samples X[n][8] with the FP16 weights */
+int kmt_mlp_forward(kmt_mlp *mlp, const float *X, int n, float *out);
+/* mean squared error and its gradient with respect to the (FP16-rounded)
+ * parameters over n samples */
+int kmt_mlp_loss_grad(kmt_mlp *mlp, const float *X, const float *y, int n, double *loss, float *grad);
+/* Adam (beta 0.9 / 0.999) over minibatches of `batch` samples in the given
+ * order, `epochs` passes; loss_history[e] = mean loss of epoch e */
+int kmt_mlp_train(kmt_mlp *mlp, const float *X, const float *y, int n, int epochs, int batch, double lr,
+                  double *loss_history);
+
 #ifdef __cplusplus
 }
 #endif

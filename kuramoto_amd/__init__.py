@@ -90,6 +90,29 @@ _SIGNATURES = {
     "kmt_time_attempts": ([_P, _I, _P, _P, _P], _I),
     "kmt_layout": ([_P, _P, _P, _P], _I),
     "kmt_autotune": ([_P, _I, _P], _I),
+    "kmt_batch_create": ([_P, _I, _I, _P, _D, _D, _I, _I, _I], _I),
+    "kmt_batch_destroy": ([_P], _I),
+    "kmt_batch_set_instance": ([_P, _I, _P, _P, _D, _D, _D], _I),
+    "kmt_batch_set_factor": ([_P, _I, _P, _P, _I], _I),
+    "kmt_batch_get_adjacency_rows": ([_P, _I, _L, _L, _P], _I),
+    "kmt_batch_begin": ([_P, _P, _I, _D, _I], _I),
+    "kmt_batch_advance": ([_P, _I, _P], _I),
+    "kmt_batch_read_order": ([_P, _I, _I, _P, _P], _I),
+    "kmt_batch_get_instance": ([_P, _I, _P, _P], _I),
+    "kmt_batch_autotune": ([_P, _I, _P, _P, _P], _I),
+    "kmt_batch_time_attempts": ([_P, _I, _P, _P, _P], _I),
+    "kmt_batch_set_vector_split": ([_P, _I], _I),
+    "kmt_batch_tune_table": ([_P, _P], _I),
+    "kmt_batch_set_proposer": ([_P, _P, _I, _I], _I),
+    "kmt_proposer_param_count": ([], _I),
+    "kmt_batch_set_attempt_log": ([_P, _I], _I),
+    "kmt_batch_read_attempt_log": ([_P, _I, _P, _P], _I),
+    "kmt_mlp_create": ([_P, _I, _P, ctypes.c_uint32], _I),
+    "kmt_mlp_destroy": ([_P], _I),
+    "kmt_mlp_get_weights": ([_P, _P, _P], _I),
+    "kmt_mlp_forward": ([_P, _P, _I, _P], _I),
+    "kmt_mlp_loss_grad": ([_P, _P, _P, _I, _P, _P], _I),
+    "kmt_mlp_train": ([_P, _P, _P, _I, _I, _I, _D, _P], _I),
     "kmt_bandwidth_probe": ([_I, ctypes.c_size_t, _I, _P, _P], _I),
 }
 
@@ -349,3 +372,205 @@ class Integrator:
         if keep_rows:
             out["rows"] = np.concatenate(rows) if rows else np.zeros((0, self.N), self.dtype)
         return out
+
+
+class BatchStatus(ctypes.Structure):
+    _fields_ = [
+        ("accepted", ctypes.c_long),
+        ("rejected", ctypes.c_long),
+        ("attempts", ctypes.c_long),
+        ("rhs_evals", ctypes.c_long),
+        ("active", ctypes.c_int),
+        ("errors", ctypes.c_int),
+        ("rows_ready", ctypes.c_int),
+        ("pad_", ctypes.c_int),
+        ("t_min", ctypes.c_double),
+        ("t_max", ctypes.c_double),
+        ("osc_steps", ctypes.c_double),
+        ("proposer_accepts", ctypes.c_long),
+        ("proposer_rejects", ctypes.c_long),
+    ]
+
+
+def proposer_param_count():
+    return load().kmt_proposer_param_count()
+
+
+class Batch:
+    """B FP32 instances advanced in lockstep (kmt_batch_*).  shared=True: one
+    adjacency for all (BF16/FP16, MFMA); shared=False: one per instance."""
+
+    def __init__(self, Ns, atol=1e-6, rtol=0.0, shared=True, storage="bf16", fixed=False, device=0):
+        self.lib = load()
+        self.Ns = [int(n) for n in Ns]
+        self.B = len(self.Ns)
+        self.ptr = ctypes.c_void_p()
+        arr = np.ascontiguousarray(self.Ns, dtype=np.int64)
+        _check(self.lib.kmt_batch_create(ctypes.byref(self.ptr), device, self.B, _ptr(arr), atol, rtol, int(shared),
+                                         STORAGE[storage], int(fixed)), "kmt_batch_create")
+
+    def close(self):
+        if self.ptr:
+            self.lib.kmt_batch_destroy(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_instance(self, b, omega, theta0, c1, t0=0.0, h0=0.01):
+        om = np.ascontiguousarray(omega, np.float32)
+        th = np.ascontiguousarray(theta0, np.float32)
+        assert om.size == self.Ns[b] and th.size == self.Ns[b]
+        _check(self.lib.kmt_batch_set_instance(self.ptr, b, _ptr(om), _ptr(th), c1, t0, h0), "kmt_batch_set_instance")
+
+    def set_factor(self, b, X, w):
+        X = np.ascontiguousarray(X, np.float32)
+        w = np.ascontiguousarray(w, np.float32)
+        _check(self.lib.kmt_batch_set_factor(self.ptr, b, _ptr(X), _ptr(w), w.size), "kmt_batch_set_factor")
+
+    def adjacency_rows(self, b, i0, i1):
+        n = self.Ns[0 if b < 0 else b]
+        out = np.zeros((i1 - i0, n), np.float32)
+        _check(self.lib.kmt_batch_get_adjacency_rows(self.ptr, b, i0, i1, _ptr(out)), "kmt_batch_get_adjacency_rows")
+        return out
+
+    def begin(self, t_eval, t1, ring_rows=256):
+        te = np.ascontiguousarray(t_eval, np.float64)
+        self.ring_rows = ring_rows
+        _check(self.lib.kmt_batch_begin(self.ptr, _ptr(te) if te.size else None, te.size, t1, ring_rows),
+               "kmt_batch_begin")
+
+    def advance(self, n):
+        st = BatchStatus()
+        _check(self.lib.kmt_batch_advance(self.ptr, n, ctypes.byref(st)), "kmt_batch_advance")
+        return st
+
+    def read_order(self, b):
+        r = np.zeros(self.ring_rows, np.float32)
+        n = ctypes.c_int()
+        _check(self.lib.kmt_batch_read_order(self.ptr, b, self.ring_rows, _ptr(r), ctypes.byref(n)),
+               "kmt_batch_read_order")
+        return r[: n.value]
+
+    def get_instance(self, b):
+        th = np.zeros(self.Ns[b], np.float32)
+        st = np.zeros(8)
+        _check(self.lib.kmt_batch_get_instance(self.ptr, b, _ptr(th), _ptr(st)), "kmt_batch_get_instance")
+        keys = ("t", "h", "accepted", "rejected", "active", "error", "proposer_accepts", "proposer_rejects")
+        return th, dict(zip(keys, st.tolist()))
+
+    def autotune(self, reps=3):
+        nt, ks, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        _check(self.lib.kmt_batch_autotune(self.ptr, reps, ctypes.byref(nt), ctypes.byref(ks), ctypes.byref(ms)),
+               "kmt_batch_autotune")
+        return nt.value, ks.value, ms.value
+
+    def tune_table(self):
+        ms = np.zeros(15)
+        _check(self.lib.kmt_batch_tune_table(self.ptr, _ptr(ms)), "kmt_batch_tune_table")
+        return ms.reshape(3, 5)
+
+    def time_attempts(self, attempts):
+        region, coup = ctypes.c_double(), ctypes.c_double()
+        st = BatchStatus()
+        _check(self.lib.kmt_batch_time_attempts(self.ptr, attempts, ctypes.byref(region), ctypes.byref(coup),
+                                                ctypes.byref(st)), "kmt_batch_time_attempts")
+        return region.value, coup.value, st
+
+    def set_vector_split(self, hi_lo=True):
+        _check(self.lib.kmt_batch_set_vector_split(self.ptr, int(hi_lo)), "kmt_batch_set_vector_split")
+
+    def set_proposer(self, weights=None, enabled=True):
+        w = None if weights is None else np.ascontiguousarray(weights, np.float16).view(np.uint16)
+        _check(self.lib.kmt_batch_set_proposer(self.ptr, _ptr(w), 0 if w is None else w.size, int(enabled)),
+               "kmt_batch_set_proposer")
+
+    def set_attempt_log(self, capacity):
+        _check(self.lib.kmt_batch_set_attempt_log(self.ptr, capacity), "kmt_batch_set_attempt_log")
+
+    def read_attempt_log(self, max_n=1 << 20):
+        rec = np.zeros((max_n, 14), np.float32)
+        n = ctypes.c_int()
+        _check(self.lib.kmt_batch_read_attempt_log(self.ptr, max_n, _ptr(rec), ctypes.byref(n)),
+               "kmt_batch_read_attempt_log")
+        return rec[: n.value]
+
+    def run(self, t_eval, t1, ring_rows=256, chunk=32, max_attempts=None):
+        self.begin(t_eval, t1, ring_rows)
+        rs = [[] for _ in range(self.B)]
+        done = 0
+        while True:
+            n = chunk if max_attempts is None else min(chunk, max_attempts - done)
+            st = self.advance(n)
+            done += n
+            for b in range(self.B):
+                rs[b].append(self.read_order(b))
+            if st.active == 0 or (max_attempts is not None and done >= max_attempts):
+                break
+        return {"r": [np.concatenate(x) for x in rs], "status": st}
+
+
+class Mlp:
+    """The proposer network on the GPU (kmt_mlp_*)."""
+
+    def __init__(self, w_init=None, seed=1, device=0):
+        self.lib = load()
+        self.ptr = ctypes.c_void_p()
+        w = None if w_init is None else np.ascontiguousarray(w_init, np.float32)
+        _check(self.lib.kmt_mlp_create(ctypes.byref(self.ptr), device, _ptr(w), seed), "kmt_mlp_create")
+        self.P = proposer_param_count()
+
+    def close(self):
+        if self.ptr:
+            self.lib.kmt_mlp_destroy(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def weights(self):
+        w32 = np.zeros(self.P, np.float32)
+        w16 = np.zeros(self.P, np.uint16)
+        _check(self.lib.kmt_mlp_get_weights(self.ptr, _ptr(w32), _ptr(w16)), "kmt_mlp_get_weights")
+        return w32, w16.view(np.float16)
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, np.float32)
+        out = np.zeros(X.shape[0], np.float32)
+        _check(self.lib.kmt_mlp_forward(self.ptr, _ptr(X), X.shape[0], _ptr(out)), "kmt_mlp_forward")
+        return out
+
+    def loss_grad(self, X, y):
+        X = np.ascontiguousarray(X, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        loss = ctypes.c_double()
+        g = np.zeros(self.P, np.float32)
+        _check(self.lib.kmt_mlp_loss_grad(self.ptr, _ptr(X), _ptr(y), X.shape[0], ctypes.byref(loss), _ptr(g)),
+               "kmt_mlp_loss_grad")
+        return loss.value, g
+
+    def train(self, X, y, epochs=10, batch=1024, lr=1e-3):
+        X = np.ascontiguousarray(X, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        hist = np.zeros(epochs)
+        _check(self.lib.kmt_mlp_train(self.ptr, _ptr(X), _ptr(y), X.shape[0], epochs, batch, lr, _ptr(hist)),
+               "kmt_mlp_train")
+        return hist

@@ -13,17 +13,60 @@
 #include <vector>
 
 #include "../../include/kuramoto_amd.h"
+#include "kmt_common.hpp"
 #include "kmt_kernels.hpp"
 #include "kmt_rng.h"
 
 using namespace kmt;
+
+namespace kmt {
+// ---------------------------------------------------------------------
+// Bandwidth probes: a plain 16-byte copy and a read-only stream of known
+// size (the measured ceiling next to the nominal 8 TB/s, and the
+// calibration kernel for the HBM byte counters).
+// ---------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void copy_probe_kernel(const u32x4 *src, u32x4 *dst, size_t n16) {
+  const size_t stride = (size_t)gridDim.x * kBlock;
+  size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  for (; i + 3 * stride < n16; i += 4 * stride) {
+    const u32x4 a = __builtin_nontemporal_load(src + i);
+    const u32x4 b = __builtin_nontemporal_load(src + i + stride);
+    const u32x4 c = __builtin_nontemporal_load(src + i + 2 * stride);
+    const u32x4 d = __builtin_nontemporal_load(src + i + 3 * stride);
+    dst[i] = a;
+    dst[i + stride] = b;
+    dst[i + 2 * stride] = c;
+    dst[i + 3 * stride] = d;
+  }
+  for (; i < n16; i += stride) dst[i] = __builtin_nontemporal_load(src + i);
+}
+
+__global__ __launch_bounds__(kBlock) void read_probe_kernel(const u32x4 *src, size_t n16, unsigned int *sink) {
+  const size_t stride = (size_t)gridDim.x * kBlock;
+  size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  unsigned int acc = 0;
+  for (; i + 3 * stride < n16; i += 4 * stride) {
+    const u32x4 a = __builtin_nontemporal_load(src + i);
+    const u32x4 b = __builtin_nontemporal_load(src + i + stride);
+    const u32x4 c = __builtin_nontemporal_load(src + i + 2 * stride);
+    const u32x4 d = __builtin_nontemporal_load(src + i + 3 * stride);
+    acc ^= a.x ^ a.y ^ a.z ^ a.w ^ b.x ^ b.y ^ b.z ^ b.w ^ c.x ^ c.y ^ c.z ^ c.w ^ d.x ^ d.y ^ d.z ^ d.w;
+  }
+  for (; i < n16; i += stride) {
+    const u32x4 v = __builtin_nontemporal_load(src + i);
+    acc ^= v.x ^ v.y ^ v.z ^ v.w;
+  }
+  if (acc == 0x9E3779B9u) sink[0] = acc;  // keeps the loads alive; practically never taken
+}
+
+}  // namespace kmt
 
 // ---------------------------------------------------------------------
 // errors
 // ---------------------------------------------------------------------
 static thread_local std::string g_last_error;
 
-static int fail(int code, const char *fmt, ...) {
+int kmt_fail(int code, const char *fmt, ...) {
   char buf[512];
   va_list ap;
   va_start(ap, fmt);
@@ -32,20 +75,7 @@ static int fail(int code, const char *fmt, ...) {
   g_last_error = buf;
   return code;
 }
-
-#define HIP_TRY(expr)                                                                           \
-  do {                                                                                          \
-    hipError_t e_ = (expr);                                                                     \
-    if (e_ != hipSuccess)                                                                       \
-      return fail(e_ == hipErrorOutOfMemory ? KMT_ERR_MEMORY : KMT_ERR_HIP, "%s failed: %s (%s:%d)", \
-                  #expr, hipGetErrorString(e_), __FILE__, __LINE__);                            \
-  } while (0)
-
-#define KMT_TRY(expr)            \
-  do {                           \
-    int rc_ = (expr);            \
-    if (rc_ != KMT_OK) return rc_; \
-  } while (0)
+#define fail kmt_fail
 
 static inline long round_up(long x, long m) { return (x + m - 1) / m * m; }
 

@@ -1,0 +1,1373 @@
+// Batched integration: B independent DP45 runs advanced in lockstep stages
+// on one GPU, each with its own step size, accept/reject decision, output
+// grid cursor and order-parameter ring (the in-process replacement for the
+// reference's SLURM arrays of single runs, job1.sh:13-25).  FP32 state.
+//
+// Two coupling layouts:
+//  * shared   — all instances have the same N and one adjacency A (BF16 or
+//               FP16 in HBM).  A stage is the product Z^T = V^T A^T on MFMA
+//               (v_mfma_f32_32x32x16_{bf16,f16}), V^T = the 2B rows
+//               (cos theta_b, sin theta_b) split into hi + lo 16-bit parts
+//               so the vectors keep ~16 bits; split-K over workgroups with
+//               fixed-order partial sums.  A is read once per stage for all
+//               instances.
+//  * grouped  — instance b has its own N_b and its own adjacency A_b; a
+//               stage is one grouped gemv-pair launch over (instance,
+//               row-block) work items (the plain stage kernel's loop).
+// After the coupling launch, one epilogue launch applies the RHS
+// (kuramoto.cu:45) and the DP45 stage combination per (instance, oscillator);
+// controller / dense output / order parameter follow the single-run kernels
+// with an instance index.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/kuramoto_amd.h"
+#include "kmt_common.hpp"
+#include "kmt_kernels.hpp"
+
+namespace kmt {
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+
+constexpr int kMaxFeat = 8;    // proposer inputs
+constexpr int kHidden = 64;    // proposer hidden width
+
+// Per-instance device state.
+struct BInst {
+  float t, h, t_last, c1;
+  float mscale;         // M = mscale * stored (per instance in grouped mode)
+  int active, commit_pending, pend_lo, pend_hi;
+  int eval_i, ring_count, error, last_accept;
+  int n;                // oscillators of this instance
+  int proposed;         // the current h came from the learned proposer
+  long long accepted, rejected, attempts, rhs_evals;
+  long long proposer_accepts, proposer_rejects;
+  double norm;
+  float logh[3], logerr[3];  // history for the proposer (most recent first)
+  float r_now, r_prev;       // order parameter at the last two accepted steps
+  float t_prev;
+  int hist_n;
+};
+
+// ---------------------------------------------------------------------
+// sin/cos of each instance's phases; optionally the hi/lo 16-bit rows of V^T
+// ---------------------------------------------------------------------
+template <class T> __device__ __forceinline__ uint16_t to16(float v);
+template <> __device__ __forceinline__ uint16_t to16<bf16_t>(float v) { return to_bf16_rne(v); }
+template <> __device__ __forceinline__ uint16_t to16<f16_t>(float v) { return __builtin_bit_cast(uint16_t, (_Float16)v); }
+template <class T> __device__ __forceinline__ float from16(uint16_t u);
+template <> __device__ __forceinline__ float from16<bf16_t>(uint16_t u) { return __builtin_bit_cast(float, (uint32_t)u << 16); }
+template <> __device__ __forceinline__ float from16<f16_t>(uint16_t u) { return (float)__builtin_bit_cast(_Float16, u); }
+
+struct VtOut {
+  uint16_t *hi, *lo;  // [Mpad][ldk], row b = cos_b, row B + b = sin_b
+  long ldk;
+  int B;
+};
+
+template <class T> __device__ __forceinline__ void put_vt(const VtOut &v, int b, long n, float c, float s) {
+  const uint16_t ch = to16<T>(c), sh = to16<T>(s);
+  v.hi[(size_t)b * v.ldk + n] = ch;
+  v.hi[(size_t)(v.B + b) * v.ldk + n] = sh;
+  v.lo[(size_t)b * v.ldk + n] = to16<T>(c - from16<T>(ch));
+  v.lo[(size_t)(v.B + b) * v.ldk + n] = to16<T>(s - from16<T>(sh));
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void bsincos_kernel(const float *y, const BInst *inst, long ldn, float *cs,
+                                                         VtOut vt, int write_vt) {
+  const int b = blockIdx.y;
+  const long n = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (n >= inst[b].n) return;
+  const float v = y[(size_t)b * ldn + n];
+  const float c = cosf(v), s = sinf(v);
+  cs[((size_t)b * 2) * ldn + n] = c;
+  cs[((size_t)b * 2 + 1) * ldn + n] = s;
+  if (write_vt) put_vt<T>(vt, b, n, c, s);
+}
+
+// ---------------------------------------------------------------------
+// Shared adjacency: Z^T (Mpad x N) = V^T (Mpad x K) . A^T (K x N) on MFMA.
+// Workgroup = 4 waves; wave w owns oscillators n0 + [0, 32 NT) and all MT
+// 32-row tiles of V^T; K is split over gridDim.y slabs.  V^T chunks
+// (MT*32 x 64 hi and lo) are staged in LDS and shared by the 4 waves; A
+// fragments (8 consecutive 16-bit values of one row per lane) come straight
+// from HBM.  Operand layouts (verified on gfx950): A-operand lane l holds
+// row l%32, k = 8(l/32)..+7; B-operand lane l holds column l%32, the same
+// k; D lane l, item v holds row 8(v/4) + 4(l/32) + v%4, column l%32.
+// ---------------------------------------------------------------------
+constexpr int kKC = 64;        // K per LDS chunk
+constexpr int kLdsRow = 72;    // padded LDS row (64 + 8 halves)
+
+template <class T> __device__ __forceinline__ f32x16_t mfma32(const u32x4 a, const u32x4 b, f32x16_t c);
+template <> __device__ __forceinline__ f32x16_t mfma32<bf16_t>(const u32x4 a, const u32x4 b, f32x16_t c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8_t, a), __builtin_bit_cast(bf16x8_t, b), c,
+                                                 0, 0, 0);
+}
+template <> __device__ __forceinline__ f32x16_t mfma32<f16_t>(const u32x4 a, const u32x4 b, f32x16_t c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8_t, a), __builtin_bit_cast(f16x8_t, b), c,
+                                                0, 0, 0);
+}
+
+template <class T, int MT, int NT, bool LO>
+__global__ __launch_bounds__(kBlock) void bgemm_kernel(const T *A, long lda, const uint16_t *vthi, const uint16_t *vtlo,
+                                                       long ldk, long kslab, float *P, long ldn, int mpad) {
+  __shared__ uint16_t sh_hi[MT * 32 * kLdsRow];
+  __shared__ uint16_t sh_lo[LO ? MT * 32 * kLdsRow : 1];
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const long n0 = (long)blockIdx.x * (4 * 32 * NT) + wave * (32 * NT);
+  const long kb = (long)blockIdx.y * kslab;
+  const long ke = std::min(kb + kslab, lda);
+  f32x16_t acc[MT][NT];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) acc[mt][nt][v] = 0.f;
+
+  for (long kc = kb; kc < ke; kc += kKC) {
+    // A fragments of this chunk: 4 k-steps of 16 per n-tile
+    u32x4 af[NT][4];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+      const T *row = A + (size_t)(n0 + nt * 32 + (lane % 32)) * lda + kc + 8 * (lane / 32);
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks) af[nt][ks] = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(row + 16 * ks));
+    }
+    // stage V^T chunk into LDS: MT*32 rows x 64 halves (8 pieces of 16 B per row)
+    for (int p = threadIdx.x; p < MT * 32 * 8; p += kBlock) {
+      const int r = p / 8, part = p % 8;
+      *reinterpret_cast<u32x4 *>(sh_hi + r * kLdsRow + part * 8) =
+          *reinterpret_cast<const u32x4 *>(vthi + (size_t)r * ldk + kc + part * 8);
+      if constexpr (LO)
+        *reinterpret_cast<u32x4 *>(sh_lo + r * kLdsRow + part * 8) =
+            *reinterpret_cast<const u32x4 *>(vtlo + (size_t)r * ldk + kc + part * 8);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt) {
+        const int off = (mt * 32 + lane % 32) * kLdsRow + 16 * ks + 8 * (lane / 32);
+        const u32x4 vh = *reinterpret_cast<const u32x4 *>(sh_hi + off);
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = mfma32<T>(vh, af[nt][ks], acc[mt][nt]);
+        if constexpr (LO) {
+          const u32x4 vl = *reinterpret_cast<const u32x4 *>(sh_lo + off);
+#pragma unroll
+          for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = mfma32<T>(vl, af[nt][ks], acc[mt][nt]);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  float *Pk = P + (size_t)blockIdx.y * mpad * ldn;
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int m = mt * 32 + 8 * (v / 4) + 4 * (lane / 32) + (v % 4);
+        Pk[(size_t)m * ldn + n0 + nt * 32 + lane % 32] = acc[mt][nt][v];
+      }
+}
+
+// ---------------------------------------------------------------------
+// Grouped adjacency: one gemv-pair per instance, work item = (b, row0).
+// The loop matches the single-run stage kernel (same summation order).
+// ---------------------------------------------------------------------
+struct GroupItem {
+  int b;
+  int row0;
+};
+
+template <class S, int ROWS>
+__global__ __launch_bounds__(kBlock) void bgemv_kernel(const S *Abase, const size_t *a_off, const long *a_ld,
+                                                       const GroupItem *items, const BInst *inst, const float *cs,
+                                                       long ldn, float *P, int B, int check_state) {
+  using Acc = typename Store<S>::Acc;
+  constexpr int VEC = Store<S>::VEC;
+  constexpr long STRIDE = (long)kBlock * VEC;
+  const GroupItem it = items[blockIdx.x];
+  const BInst *in = inst + it.b;
+  if (check_state && (!in->active || in->commit_pending)) return;
+  const long lda = a_ld[it.b];
+  const S *Ab = Abase + a_off[it.b] + (size_t)it.row0 * lda;
+  const float *cosv = cs + (size_t)it.b * 2 * ldn;
+  const float *sinv = cosv + ldn;
+  const int tid = threadIdx.x;
+  Acc accC[ROWS], accS[ROWS];
+#pragma unroll
+  for (int r = 0; r < ROWS; ++r) { accC[r] = 0; accS[r] = 0; }
+  long c = (long)tid * VEC;
+  for (; c + STRIDE < lda; c += 2 * STRIDE) {
+    float cv0[VEC], sv0[VEC], cv1[VEC], sv1[VEC];
+    u32x4 raw0[ROWS], raw1[ROWS];
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) raw0[r] = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(Ab + r * lda + c));
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r)
+      raw1[r] = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(Ab + r * lda + c + STRIDE));
+    load_reals<float, VEC>(cosv + c, cv0);
+    load_reals<float, VEC>(sinv + c, sv0);
+    load_reals<float, VEC>(cosv + c + STRIDE, cv1);
+    load_reals<float, VEC>(sinv + c + STRIDE, sv1);
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+      Acc m[VEC];
+      unpack<S>(raw0[r], m);
+#pragma unroll
+      for (int q = 0; q < VEC; ++q) {
+        accC[r] = rfma<Acc>(m[q], cv0[q], accC[r]);
+        accS[r] = rfma<Acc>(m[q], sv0[q], accS[r]);
+      }
+      unpack<S>(raw1[r], m);
+#pragma unroll
+      for (int q = 0; q < VEC; ++q) {
+        accC[r] = rfma<Acc>(m[q], cv1[q], accC[r]);
+        accS[r] = rfma<Acc>(m[q], sv1[q], accS[r]);
+      }
+    }
+  }
+  if (c < lda) {
+    float cv0[VEC], sv0[VEC];
+    u32x4 raw0[ROWS];
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) raw0[r] = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(Ab + r * lda + c));
+    load_reals<float, VEC>(cosv + c, cv0);
+    load_reals<float, VEC>(sinv + c, sv0);
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+      Acc m[VEC];
+      unpack<S>(raw0[r], m);
+#pragma unroll
+      for (int q = 0; q < VEC; ++q) {
+        accC[r] = rfma<Acc>(m[q], cv0[q], accC[r]);
+        accS[r] = rfma<Acc>(m[q], sv0[q], accS[r]);
+      }
+    }
+  }
+  __shared__ Acc red[kBlock / kWave][2 * ROWS];
+  const int lane = tid & (kWave - 1), wave = tid / kWave;
+#pragma unroll
+  for (int r = 0; r < ROWS; ++r) {
+    accC[r] = wave_sum(accC[r]);
+    accS[r] = wave_sum(accS[r]);
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) { red[wave][r] = accC[r]; red[wave][ROWS + r] = accS[r]; }
+  }
+  __syncthreads();
+  if (tid < ROWS) {
+    const long i = it.row0 + tid;
+    if (i < in->n) {
+      Acc C = red[0][tid], Sv = red[0][ROWS + tid];
+#pragma unroll
+      for (int w = 1; w < kBlock / kWave; ++w) { C += red[w][tid]; Sv += red[w][ROWS + tid]; }
+      P[(size_t)it.b * ldn + i] = C;
+      P[(size_t)(B + it.b) * ldn + i] = Sv;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------
+// Epilogue: RHS + DP45 stage combination per (instance, oscillator).
+// ---------------------------------------------------------------------
+struct BVecs {
+  const float *cs_in;
+  float *cs_out;
+  long ldn;
+  int B;
+  const float *omega;
+  float *y, *ynew;
+  float *kbuf;        // [7][B][ldn]
+  float *f_out;       // RHS kind output, [B][ldn]
+  const float *P;     // [ksplit][mrows][ldn]
+  int ksplit, mrows;
+  BInst *inst;
+  double *err_part;   // [B][nblk]
+  int nblk;
+  float atl, rtl;
+  VtOut vt;
+  int write_vt;
+};
+
+template <class T, int KIND>
+__global__ __launch_bounds__(kBlock) void bepilogue_kernel(const BVecs a) {
+  const int b = blockIdx.y;
+  BInst *in = a.inst + b;
+  if (KIND >= 2 && (!in->active || in->commit_pending)) return;
+  __shared__ double sh[kBlock];
+  const long n = (long)blockIdx.x * kBlock + threadIdx.x;
+  const size_t vb = (size_t)b * a.ldn;
+  const size_t kst = (size_t)a.B * a.ldn;  // stride between k_j slabs
+  double sq = 0.0;
+  if (n < in->n) {
+    float C = 0.f, S = 0.f;
+    for (int ks = 0; ks < a.ksplit; ++ks) {
+      C += a.P[((size_t)ks * a.mrows + b) * a.ldn + n];
+      S += a.P[((size_t)ks * a.mrows + a.B + b) * a.ldn + n];
+    }
+    C *= in->mscale;
+    S *= in->mscale;
+    const float cs = a.cs_in[2 * vb + n], sn = a.cs_in[2 * vb + a.ldn + n];
+    const float f = rhs_value<float>(a.omega[vb + n], in->c1, cs, sn, C, S);
+    const float h = in->h;
+    if constexpr (KIND == KIND_RHS) {
+      a.f_out[vb + n] = f;
+    } else if constexpr (KIND < 7) {
+      constexpr int s = KIND;
+      a.kbuf[(s - 1) * kst + vb + n] = f;
+      float acc = a.kbuf[vb + n] * (float)tab::A[s - 1][0];
+#pragma unroll
+      for (int j = 1; j < s; ++j) {
+        const float kj = (j == s - 1) ? f : a.kbuf[j * kst + vb + n];
+        acc = rfma<float>(kj, (float)tab::A[s - 1][j], acc);
+      }
+      const float yt = rfma<float>(acc, h, a.y[vb + n]);
+      if constexpr (s + 1 == 7) a.ynew[vb + n] = yt;
+      const float c = cosf(yt), si = sinf(yt);
+      a.cs_out[2 * vb + n] = c;
+      a.cs_out[2 * vb + a.ldn + n] = si;
+      if (a.write_vt) put_vt<T>(a.vt, b, n, c, si);
+    } else {
+      a.kbuf[6 * kst + vb + n] = f;
+      float acc = a.kbuf[vb + n] * (float)tab::E[0];
+#pragma unroll
+      for (int j = 1; j < 7; ++j) {
+        const float kj = (j == 6) ? f : a.kbuf[j * kst + vb + n];
+        acc = rfma<float>(kj, (float)tab::E[j], acc);
+      }
+      const float num = h * acc;
+      const float den = rfma<float>(a.rtl, fmaxf(fabsf(a.y[vb + n]), fabsf(a.ynew[vb + n])), a.atl);
+      const float e = num / den;
+      sq = (double)e * (double)e;
+    }
+  }
+  if constexpr (KIND == 7) {
+    const double tot = block_sum(sq, sh);
+    if (threadIdx.x == 0) a.err_part[(size_t)b * a.nblk + blockIdx.x] = tot;
+  }
+}
+
+// ---------------------------------------------------------------------
+// Learned step-size proposer: MLP 8 -> 64 -> 64 -> 1 (tanh, tanh, linear),
+// FP16 weights, FP32 accumulation.  Output = log(h_next / h_standard), i.e.
+// a correction of the standard controller's proposal, clamped to [-0.7, 0.7].
+// ---------------------------------------------------------------------
+struct MlpW {
+  const uint16_t *w;  // FP16 parameters, packed: W1[64][8], b1[64], W2[64][64], b2[64], W3[64], b3
+  int enabled;
+};
+constexpr int kMlpParams = kHidden * kMaxFeat + kHidden + kHidden * kHidden + kHidden + kHidden + 1;
+
+__device__ __forceinline__ float h2f(uint16_t u) { return (float)__builtin_bit_cast(_Float16, u); }
+
+// one thread evaluates the network (called by thread 0 of the controller)
+__device__ float mlp_eval(const uint16_t *w, const float *x) {
+  const uint16_t *W1 = w, *b1 = W1 + kHidden * kMaxFeat, *W2 = b1 + kHidden, *b2 = W2 + kHidden * kHidden,
+                 *W3 = b2 + kHidden, *b3 = W3 + kHidden;
+  float h1[kHidden];
+  for (int i = 0; i < kHidden; ++i) {
+    float a = h2f(b1[i]);
+    for (int j = 0; j < kMaxFeat; ++j) a = fmaf(h2f(W1[i * kMaxFeat + j]), x[j], a);
+    h1[i] = tanhf(a);
+  }
+  float out = h2f(b3[0]);
+  for (int i = 0; i < kHidden; ++i) {
+    float a = h2f(b2[i]);
+    for (int j = 0; j < kHidden; ++j) a = fmaf(h2f(W2[i * kHidden + j]), h1[j], a);
+    out = fmaf(h2f(W3[i]), tanhf(a), out);
+  }
+  return fminf(fmaxf(out, -0.7f), 0.7f);
+}
+
+__device__ void proposer_features(const BInst *in, float h_std, float *x) {
+  // last 3 log h (relative to the standard proposal), last 3 log error ratios,
+  // r and dr/dt (the north_star's inputs)
+  const float lh = logf(h_std);
+  for (int q = 0; q < 3; ++q) x[q] = in->hist_n > q ? in->logh[q] - lh : 0.f;
+  for (int q = 0; q < 3; ++q) x[3 + q] = in->hist_n > q ? in->logerr[q] : 0.f;
+  x[6] = in->r_now;
+  const float dtp = in->t - in->t_prev;
+  x[7] = (in->hist_n > 1 && dtp > 0.f) ? fminf(fmaxf((in->r_now - in->r_prev) / dtp, -10.f), 10.f) : 0.f;
+}
+
+struct LogRec {   // one accepted/rejected attempt for training the proposer
+  float x[kMaxFeat];
+  float h_used, h_std, err_norm;
+  int accepted, b;
+  int after_accept;  // h came from the proposal made after an accepted step
+};
+
+// Controller per instance (dp45.cu:117-138), block b.  With the proposer on,
+// the next h after an accepted step is h_std * exp(mlp(x)); after a
+// rejection the standard controller is used (fallback).
+__global__ __launch_bounds__(kBlock) void bcontrol_kernel(BInst *inst, const double *err_part, int nblk, float t1,
+                                                          int fixed, const float *t_eval, int n_eval, MlpW mlp,
+                                                          LogRec *log, int *log_count, int log_cap) {
+  const int b = blockIdx.x;
+  BInst *in = inst + b;
+  if (!in->active || in->commit_pending) return;
+  __shared__ double sh[kBlock];
+  double s = 0.0;
+  for (int q = threadIdx.x; q < nblk; q += kBlock) s += err_part[(size_t)b * nblk + q];
+  const double sum = block_sum(s, sh);
+  if (threadIdx.x != 0) return;
+  in->attempts += 1;
+  in->rhs_evals += 6;
+  const float t = in->t, h = in->h;
+  int accept;
+  float hstd = h, norm = 0.f;
+  if (fixed) {
+    accept = 1;
+  } else {
+    const float nrm = (float)sqrt(sum);
+    norm = (float)((double)nrm / sqrt((double)in->n));
+    float factor = (float)(0.9 * pow((double)norm, -0.2));
+    in->norm = (double)norm;
+    if (norm < 1.f) {
+      accept = 1;
+      if (factor > 10.f) factor = 10.f;
+      hstd = h * factor;
+    } else {
+      accept = 0;
+      if (factor < 1.f) {
+        if ((double)factor < 0.2) factor = 0.2f;
+        hstd = h * factor;
+      }
+      if (isnan(norm)) {
+        in->error = 1;
+        in->active = 0;
+      }
+    }
+  }
+  // training log of every attempt (features as seen before this attempt)
+  if (log && !fixed) {
+    const int slot = atomicAdd(log_count, 1);
+    if (slot < log_cap) {
+      LogRec &L = log[slot];
+      proposer_features(in, h, L.x);
+      L.h_used = h;
+      L.h_std = h;
+      L.err_norm = norm;
+      L.accepted = accept;
+      L.b = b;
+      L.after_accept = in->last_accept;
+    }
+  }
+  if (in->proposed) {
+    if (accept) in->proposer_accepts += 1;
+    else in->proposer_rejects += 1;
+  }
+  in->last_accept = accept;
+  if (!accept) {
+    in->rejected += 1;
+    in->h = hstd;
+    in->proposed = 0;
+    return;
+  }
+  // history for the proposer
+  in->logh[2] = in->logh[1];
+  in->logh[1] = in->logh[0];
+  in->logh[0] = logf(h);
+  in->logerr[2] = in->logerr[1];
+  in->logerr[1] = in->logerr[0];
+  in->logerr[0] = logf(fmaxf(norm, 1e-12f));
+  in->hist_n = min(in->hist_n + 1, 3);
+  const float tn = t + h;
+  in->t_prev = t;
+  in->t_last = t;
+  in->t = tn;
+  float hnext = hstd;
+  in->proposed = 0;
+  if (mlp.enabled && !fixed && in->hist_n >= 3) {
+    float x[kMaxFeat];
+    proposer_features(in, hstd, x);
+    hnext = hstd * expf(mlp_eval(mlp.w, x));
+    in->proposed = 1;
+  }
+  in->h = hnext;
+  in->accepted += 1;
+  int j = in->eval_i;
+  while (j < n_eval && tn >= t_eval[j]) ++j;
+  in->pend_lo = in->eval_i;
+  in->pend_hi = j;
+  in->commit_pending = 1;
+}
+
+// Dense output (order parameter only), commit, next stage-2 vector.
+struct BEmit {
+  BInst *inst;
+  long ldn;
+  int B;
+  float *y, *ylast, *ynew, *kbuf;
+  float *cs_out;
+  const float *t_eval;
+  double *ord_part;   // [B][ring_cap][nblk][2]
+  int ring_cap, nblk;
+  VtOut vt;
+  int write_vt;
+};
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void bemit_kernel(const BEmit a) {
+  const int b = blockIdx.y;
+  const BInst *in = a.inst + b;
+  if (!in->active) return;
+  __shared__ double sh[kBlock];
+  const long n = (long)blockIdx.x * kBlock + threadIdx.x;
+  const bool valid = n < in->n;
+  const size_t vb = (size_t)b * a.ldn;
+  const size_t kst = (size_t)a.B * a.ldn;
+  const float h = in->h;
+  auto stage2 = [&](float yv, float k1) {
+    const float yt = rfma<float>(k1 * (float)tab::A[0][0], h, yv);
+    const float c = cosf(yt), s = sinf(yt);
+    a.cs_out[2 * vb + n] = c;
+    a.cs_out[2 * vb + a.ldn + n] = s;
+    if (a.write_vt) put_vt<T>(a.vt, b, n, c, s);
+  };
+  if (!in->commit_pending) {
+    if (valid) stage2(a.y[vb + n], a.kbuf[vb + n]);
+    return;
+  }
+  const int pend = in->pend_hi - in->pend_lo;
+  const int room = a.ring_cap - in->ring_count;
+  const int n_emit = pend < room ? pend : room;
+  if (n_emit > 0) {
+    const float t = in->t, tl = in->t_last, hh = t - tl;
+    float Q[4] = {0, 0, 0, 0}, yl = 0;
+    if (valid) {
+      float kv[7];
+#pragma unroll
+      for (int j = 0; j < 7; ++j) kv[j] = a.kbuf[j * kst + vb + n];
+      dense_coeffs<float>(kv, Q);
+      yl = a.y[vb + n];
+    }
+    for (int q = 0; q < n_emit; ++q) {
+      const int slot = in->ring_count + q;
+      const float x = (a.t_eval[in->pend_lo + q] - tl) / hh;
+      double cx = 0, sx = 0;
+      if (valid) {
+        const float val = dense_value<float>(Q, yl, hh, x);
+        cx = (double)cosf(val);
+        sx = (double)sinf(val);
+      }
+      const double bc = block_sum(cx, sh), bs = block_sum(sx, sh);
+      if (threadIdx.x == 0) {
+        const size_t o = (((size_t)b * a.ring_cap + slot) * a.nblk + blockIdx.x) * 2;
+        a.ord_part[o] = bc;
+        a.ord_part[o + 1] = bs;
+      }
+    }
+  }
+  if (n_emit == pend && valid) {
+    const float yo = a.y[vb + n], yn = a.ynew[vb + n], k7 = a.kbuf[6 * kst + vb + n];
+    a.ylast[vb + n] = yo;
+    a.y[vb + n] = yn;
+    a.kbuf[vb + n] = k7;
+    stage2(yn, k7);
+  }
+}
+
+// r for the emitted rows, ring bookkeeping, end of run per instance; also
+// the order parameter at the accepted step for the proposer's features
+// (the last emitted output's r).
+__global__ __launch_bounds__(kBlock) void bfinalize_kernel(BInst *inst, const double *ord_part, int nblk, int ring_cap,
+                                                           float t1, float *r_ring) {
+  const int b = blockIdx.x;
+  BInst *in = inst + b;
+  if (!in->active || !in->commit_pending) return;
+  __shared__ double sh[kBlock];
+  const int pend = in->pend_hi - in->pend_lo;
+  const int room = ring_cap - in->ring_count;
+  const int n_emit = pend < room ? pend : room;
+  const int base = in->ring_count;
+  float r_last = in->r_now;
+  for (int q = 0; q < n_emit; ++q) {
+    const int slot = base + q;
+    double sx = 0, sy = 0;
+    for (int x = threadIdx.x; x < nblk; x += kBlock) {
+      const size_t o = (((size_t)b * ring_cap + slot) * nblk + x) * 2;
+      sx += ord_part[o];
+      sy += ord_part[o + 1];
+    }
+    const double X = block_sum(sx, sh), Y = block_sum(sy, sh);
+    const float r = order_value<float>(X, Y, in->n);
+    if (threadIdx.x == 0) r_ring[(size_t)b * ring_cap + slot] = r;
+    r_last = r;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (n_emit > 0) {
+      in->r_prev = in->r_now;
+      in->r_now = r_last;
+    }
+    in->ring_count = base + n_emit;
+    in->pend_lo += n_emit;
+    if (in->pend_lo == in->pend_hi) {
+      in->commit_pending = 0;
+      in->eval_i = in->pend_hi;
+      if (!(in->t < t1)) in->active = 0;
+    }
+  }
+}
+
+}  // namespace kmt
+
+using namespace kmt;
+
+static inline long rup(long x, long m) { return (x + m - 1) / m * m; }
+
+// ---------------------------------------------------------------------
+// Engine
+// ---------------------------------------------------------------------
+struct kmt_batch {
+  int device = 0, B = 0, shared = 0, storage = KMT_ADJ_BF16, fixed = 0;
+  float atl = 1e-6f, rtl = 0.f;
+  std::vector<long> Nb;
+  long Nmax = 0, ldn = 0, lda_shared = 0, rows_shared = 0;
+  int mpad = 32, mt = 1, nblk = 0;
+  hipStream_t stream = nullptr;
+  float *y = nullptr, *ylast = nullptr, *ynew = nullptr, *om = nullptr, *kbuf = nullptr, *cs[2] = {nullptr, nullptr};
+  float *P = nullptr;
+  size_t P_elems = 0;
+  uint16_t *vthi = nullptr, *vtlo = nullptr;
+  double *err_part = nullptr, *ord_part = nullptr;
+  float *r_ring = nullptr, *t_eval = nullptr;
+  BInst *inst = nullptr;
+  std::vector<BInst> host_inst;
+  void *A = nullptr;
+  size_t A_bytes = 0;
+  std::vector<size_t> a_off;
+  std::vector<long> a_ld;
+  std::vector<float> mscale;
+  size_t *a_off_d = nullptr;
+  long *a_ld_d = nullptr;
+  GroupItem *items = nullptr;
+  int n_items = 0;
+  int n_eval = 0, ring_cap = 0;
+  float t1 = 0.f;
+  int begun = 0;
+  // tiling (shared) / rows (grouped)
+  int tile_nt = 1, ksplit = 4, lo = 1;
+  double tune_ms[16] = {0};  // (nt in 1,2,4) x (split in 1,2,4,8,16)
+  hipGraphExec_t g1 = nullptr, g8 = nullptr;
+  // proposer
+  uint16_t *mlp_w = nullptr;
+  int mlp_on = 0;
+  LogRec *log = nullptr;
+  int *log_count = nullptr;
+  int log_cap = 0;
+
+  ~kmt_batch() { release(); }
+  void drop_graphs() {
+    if (g1) hipGraphExecDestroy(g1);
+    if (g8) hipGraphExecDestroy(g8);
+    g1 = g8 = nullptr;
+  }
+  void release() {
+    if (stream) hipStreamSynchronize(stream);
+    drop_graphs();
+    void *ptrs[] = {y, ylast, ynew, om, kbuf, cs[0], cs[1], P, vthi, vtlo, err_part, ord_part, r_ring, t_eval,
+                    inst, A, a_off_d, a_ld_d, items, mlp_w, log, log_count};
+    for (void *p : ptrs)
+      if (p) hipFree(p);
+    if (stream) hipStreamDestroy(stream);
+    stream = nullptr;
+  }
+  template <class T> int alloc(T **p, size_t n) {
+    HIP_TRY(hipMalloc((void **)p, sizeof(T) * std::max<size_t>(n, 1)));
+    HIP_TRY(hipMemsetAsync(*p, 0, sizeof(T) * std::max<size_t>(n, 1), stream));
+    return KMT_OK;
+  }
+
+  int create(int dev, int nb, const long *N, double a, double r, int shr, int stg, int fx) {
+    device = dev;
+    B = nb;
+    shared = shr;
+    storage = stg;
+    fixed = fx;
+    atl = (float)a;
+    rtl = (float)r;
+    Nb.assign(N, N + B);
+    Nmax = *std::max_element(Nb.begin(), Nb.end());
+    if (shared)
+      for (long n : Nb)
+        if (n != Nb[0]) return kmt_fail(KMT_ERR_ARG, "shared adjacency needs equal N for all instances");
+    if (shared && storage == KMT_ADJ_NATIVE) return kmt_fail(KMT_ERR_ARG, "shared (MFMA) batches store A in bf16 or f16");
+    if (shared && 2 * B > 128) return kmt_fail(KMT_ERR_ARG, "shared batches hold at most 64 instances per context");
+    HIP_TRY(hipSetDevice(device));
+    HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    ldn = rup(Nmax, shr ? 512 : 256);
+    nblk = (int)(ldn / kBlock);
+    KMT_TRY(alloc(&y, (size_t)B * ldn));
+    KMT_TRY(alloc(&ylast, (size_t)B * ldn));
+    KMT_TRY(alloc(&ynew, (size_t)B * ldn));
+    KMT_TRY(alloc(&om, (size_t)B * ldn));
+    KMT_TRY(alloc(&kbuf, (size_t)7 * B * ldn));
+    KMT_TRY(alloc(&cs[0], (size_t)2 * B * ldn));
+    KMT_TRY(alloc(&cs[1], (size_t)2 * B * ldn));
+    KMT_TRY(alloc(&err_part, (size_t)B * nblk));
+    KMT_TRY(alloc(&inst, (size_t)B));
+    host_inst.assign(B, BInst{});
+    for (int b = 0; b < B; ++b) {
+      std::memset(&host_inst[b], 0, sizeof(BInst));
+      host_inst[b].n = (int)Nb[b];
+      host_inst[b].mscale = 1.f;
+    }
+    mscale.assign(B, 1.f);
+    if (shared) {
+      mt = (2 * B + 31) / 32;
+      if (mt == 3) mt = 4;
+      mpad = 32 * mt;
+      lda_shared = rup(Nmax, 64);
+      rows_shared = ldn;  // multiple of 256 >= n0 + 4*32*NT of the last workgroup
+      KMT_TRY(alloc(&vthi, (size_t)mpad * lda_shared));
+      KMT_TRY(alloc(&vtlo, (size_t)mpad * lda_shared));
+    }
+    HIP_TRY(hipStreamSynchronize(stream));
+    return KMT_OK;
+  }
+
+  int set_instance(int b, const float *omega, const float *theta, double c1, double t0, double h0) {
+    if (b < 0 || b >= B) return kmt_fail(KMT_ERR_ARG, "instance %d out of range", b);
+    HIP_TRY(hipMemcpyAsync(om + (size_t)b * ldn, omega, sizeof(float) * Nb[b], hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipMemcpyAsync(y + (size_t)b * ldn, theta, sizeof(float) * Nb[b], hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    host_inst[b].c1 = (float)c1;
+    host_inst[b].t = (float)t0;
+    host_inst[b].h = (float)h0;
+    begun = 0;
+    return KMT_OK;
+  }
+
+  size_t ebytes() const { return storage == KMT_ADJ_NATIVE ? 4 : 2; }
+
+  int ensure_A() {
+    if (A) return KMT_OK;
+    a_off.assign(B, 0);
+    a_ld.assign(B, 0);
+    size_t total = 0;
+    if (shared) {
+      total = (size_t)rows_shared * lda_shared;
+      for (int b = 0; b < B; ++b) a_ld[b] = lda_shared;
+    } else {
+      for (int b = 0; b < B; ++b) {
+        a_off[b] = total;
+        a_ld[b] = rup(Nb[b], 64);
+        total += (size_t)rup(Nb[b], 64) * a_ld[b];
+      }
+    }
+    A_bytes = total * ebytes();
+    hipError_t e = hipMalloc(&A, A_bytes);
+    if (e != hipSuccess) {
+      A = nullptr;
+      return kmt_fail(KMT_ERR_MEMORY, "cannot allocate %zu bytes of adjacency: %s", A_bytes, hipGetErrorString(e));
+    }
+    HIP_TRY(hipMemsetAsync(A, 0, A_bytes, stream));
+    KMT_TRY(alloc(&a_off_d, (size_t)B));
+    KMT_TRY(alloc(&a_ld_d, (size_t)B));
+    HIP_TRY(hipMemcpyAsync(a_off_d, a_off.data(), sizeof(size_t) * B, hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipMemcpyAsync(a_ld_d, a_ld.data(), sizeof(long) * B, hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    return KMT_OK;
+  }
+
+  // dense adjacency of instance b (or the shared one, b = -1) from a factor
+  int set_factor(int b, const float *Xh, const float *wh, int K) {
+    if (shared ? b != -1 : (b < 0 || b >= B)) return kmt_fail(KMT_ERR_ARG, "instance %d: use -1 for the shared adjacency", b);
+    KMT_TRY(ensure_A());
+    const long N = shared ? Nmax : Nb[b];
+    double mx = 0;
+    for (long n = 0; n < N; ++n) {
+      double s = 0;
+      for (int m = 0; m < K; ++m) s += std::fabs((double)wh[m]) * (double)Xh[n * K + m] * (double)Xh[n * K + m];
+      mx = std::max(mx, s);
+    }
+    const double sc = (storage == KMT_ADJ_NATIVE || !(mx > 0)) ? 1.0 : std::ldexp(1.0, (int)std::ceil(std::log2(mx)));
+    float *Xd = nullptr, *wd = nullptr;
+    KMT_TRY(alloc(&Xd, (size_t)N * std::max(K, 1)));
+    KMT_TRY(alloc(&wd, (size_t)std::max(K, 1)));
+    hipError_t e = hipMemcpyAsync(Xd, Xh, sizeof(float) * N * K, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(wd, wh, sizeof(float) * K, hipMemcpyHostToDevice, stream);
+    const int bi = shared ? 0 : b;
+    const long lda = a_ld[bi];
+    const float inv = (float)(1.0 / sc);
+    for (long r0 = 0; r0 < N && e == hipSuccess; r0 += 4096) {
+      const long nr = std::min<long>(4096, N - r0);
+      if (storage == KMT_ADJ_BF16)
+        gen_adj_kernel<float, bf16_t><<<(unsigned)nr, kBlock, 0, stream>>>(Xd, wd, K, N, r0, (bf16_t *)A + a_off[bi] + (size_t)r0 * lda, lda, inv);
+      else if (storage == KMT_ADJ_F16)
+        gen_adj_kernel<float, f16_t><<<(unsigned)nr, kBlock, 0, stream>>>(Xd, wd, K, N, r0, (f16_t *)A + a_off[bi] + (size_t)r0 * lda, lda, inv);
+      else
+        gen_adj_kernel<float, float><<<(unsigned)nr, kBlock, 0, stream>>>(Xd, wd, K, N, r0, (float *)A + a_off[bi] + (size_t)r0 * lda, lda, 1.f);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    hipFree(Xd);
+    hipFree(wd);
+    if (e != hipSuccess) return kmt_fail(KMT_ERR_HIP, "adjacency build: %s", hipGetErrorString(e));
+    if (shared)
+      for (int q = 0; q < B; ++q) { mscale[q] = (float)sc; host_inst[q].mscale = (float)sc; }
+    else { mscale[b] = (float)sc; host_inst[b].mscale = (float)sc; }
+    drop_graphs();
+    return KMT_OK;
+  }
+
+  int get_adjacency_rows(int b, long i0, long i1, float *out) {
+    if (!A) return kmt_fail(KMT_ERR_STATE, "no adjacency");
+    const int bi = shared ? 0 : b;
+    const long N = shared ? Nmax : Nb[b];
+    if (i0 < 0 || i1 > N || i0 > i1) return kmt_fail(KMT_ERR_ARG, "bad row range");
+    const long lda = a_ld[bi];
+    float *stage = nullptr;
+    KMT_TRY(alloc(&stage, (size_t)(i1 - i0) * N));
+    dim3 grid((unsigned)std::min<long>((N + kBlock - 1) / kBlock, 64), (unsigned)(i1 - i0));
+    if (storage == KMT_ADJ_BF16)
+      read_rows_kernel<float, bf16_t><<<grid, kBlock, 0, stream>>>((const bf16_t *)A + a_off[bi] + (size_t)i0 * lda, lda, N, i1 - i0, mscale[bi], stage);
+    else if (storage == KMT_ADJ_F16)
+      read_rows_kernel<float, f16_t><<<grid, kBlock, 0, stream>>>((const f16_t *)A + a_off[bi] + (size_t)i0 * lda, lda, N, i1 - i0, mscale[bi], stage);
+    else
+      read_rows_kernel<float, float><<<grid, kBlock, 0, stream>>>((const float *)A + a_off[bi] + (size_t)i0 * lda, lda, N, i1 - i0, 1.f, stage);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(out, stage, sizeof(float) * (i1 - i0) * N, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    hipFree(stage);
+    if (e != hipSuccess) return kmt_fail(KMT_ERR_HIP, "adjacency read: %s", hipGetErrorString(e));
+    return KMT_OK;
+  }
+
+  VtOut vt_out() const {
+    VtOut v;
+    v.hi = vthi;
+    v.lo = vtlo;
+    v.ldk = lda_shared;
+    v.B = B;
+    return v;
+  }
+
+  // ---- coupling launches ----
+  template <class T, int MT, int NT, bool LO> void gemm_v() {
+    const long tn = 4 * 32 * NT;
+    const long kslab = rup((lda_shared + ksplit - 1) / ksplit, kKC);
+    dim3 grid((unsigned)((Nmax + tn - 1) / tn), (unsigned)ksplit);
+    bgemm_kernel<T, MT, NT, LO><<<grid, kBlock, 0, stream>>>((const T *)A, lda_shared, vthi, vtlo, lda_shared, kslab, P, ldn, mpad);
+  }
+  template <class T, int MT> void gemm_mt() {
+    if (tile_nt == 4) {
+      if (lo) gemm_v<T, MT, 4, true>(); else gemm_v<T, MT, 4, false>();
+    } else if (tile_nt == 2) {
+      if (lo) gemm_v<T, MT, 2, true>(); else gemm_v<T, MT, 2, false>();
+    } else {
+      if (lo) gemm_v<T, MT, 1, true>(); else gemm_v<T, MT, 1, false>();
+    }
+  }
+  template <class T> void gemm_t() {
+    if (mt == 1) gemm_mt<T, 1>();
+    else if (mt == 2) gemm_mt<T, 2>();
+    else gemm_mt<T, 4>();
+  }
+  void coupling(int check) {
+    if (shared) {
+      if (storage == KMT_ADJ_BF16) gemm_t<bf16_t>(); else gemm_t<f16_t>();
+    } else {
+      if (storage == KMT_ADJ_BF16)
+        bgemv_kernel<bf16_t, 8><<<n_items, kBlock, 0, stream>>>((const bf16_t *)A, a_off_d, a_ld_d, items, inst, cs_cur, ldn, P, B, check);
+      else if (storage == KMT_ADJ_F16)
+        bgemv_kernel<f16_t, 8><<<n_items, kBlock, 0, stream>>>((const f16_t *)A, a_off_d, a_ld_d, items, inst, cs_cur, ldn, P, B, check);
+      else
+        bgemv_kernel<float, 8><<<n_items, kBlock, 0, stream>>>((const float *)A, a_off_d, a_ld_d, items, inst, cs_cur, ldn, P, B, check);
+    }
+  }
+  const float *cs_cur = nullptr;
+
+  BVecs bvecs(const float *in, float *out) {
+    BVecs v;
+    v.cs_in = in;
+    v.cs_out = out;
+    v.ldn = ldn;
+    v.B = B;
+    v.omega = om;
+    v.y = y;
+    v.ynew = ynew;
+    v.kbuf = kbuf;
+    v.f_out = kbuf;  // the RHS kind writes k1
+    v.P = P;
+    v.ksplit = shared ? ksplit : 1;
+    v.mrows = shared ? mpad : 2 * B;
+    v.inst = inst;
+    v.err_part = err_part;
+    v.nblk = nblk;
+    v.atl = atl;
+    v.rtl = rtl;
+    v.vt = vt_out();
+    v.write_vt = shared;
+    return v;
+  }
+
+  template <class T> void epi(int kind, const BVecs &v) {
+    dim3 grid((unsigned)nblk, (unsigned)B);
+    switch (kind) {
+      case 0: bepilogue_kernel<T, 0><<<grid, kBlock, 0, stream>>>(v); break;
+      case 2: bepilogue_kernel<T, 2><<<grid, kBlock, 0, stream>>>(v); break;
+      case 3: bepilogue_kernel<T, 3><<<grid, kBlock, 0, stream>>>(v); break;
+      case 4: bepilogue_kernel<T, 4><<<grid, kBlock, 0, stream>>>(v); break;
+      case 5: bepilogue_kernel<T, 5><<<grid, kBlock, 0, stream>>>(v); break;
+      case 6: bepilogue_kernel<T, 6><<<grid, kBlock, 0, stream>>>(v); break;
+      default: bepilogue_kernel<T, 7><<<grid, kBlock, 0, stream>>>(v); break;
+    }
+  }
+  void epilogue(int kind, const BVecs &v) {
+    if (storage == KMT_ADJ_F16) epi<f16_t>(kind, v);
+    else epi<bf16_t>(kind, v);
+  }
+
+  BEmit bemit_args() {
+    BEmit e;
+    e.inst = inst;
+    e.ldn = ldn;
+    e.B = B;
+    e.y = y;
+    e.ylast = ylast;
+    e.ynew = ynew;
+    e.kbuf = kbuf;
+    e.cs_out = cs[0];
+    e.t_eval = t_eval;
+    e.ord_part = ord_part;
+    e.ring_cap = ring_cap;
+    e.nblk = nblk;
+    e.vt = vt_out();
+    e.write_vt = shared;
+    return e;
+  }
+  void emit() {
+    dim3 grid((unsigned)nblk, (unsigned)B);
+    if (storage == KMT_ADJ_F16) bemit_kernel<f16_t><<<grid, kBlock, 0, stream>>>(bemit_args());
+    else bemit_kernel<bf16_t><<<grid, kBlock, 0, stream>>>(bemit_args());
+  }
+  MlpW mlpw() const {
+    MlpW m;
+    m.w = mlp_w;
+    m.enabled = mlp_on && mlp_w;
+    return m;
+  }
+
+  void launch_attempt(std::vector<std::pair<hipEvent_t, hipEvent_t>> *ev = nullptr) {
+    for (int s = 2; s <= 7; ++s) {
+      const float *in = cs[(s - 2) & 1];
+      float *out = cs[(s - 1) & 1];
+      cs_cur = in;
+      if (ev) hipEventRecord((*ev)[0].first, stream);
+      coupling(1);
+      if (ev) {
+        hipEventRecord((*ev)[0].second, stream);
+        ev->erase(ev->begin());
+      }
+      epilogue(s, bvecs(in, out));
+    }
+    bcontrol_kernel<<<B, kBlock, 0, stream>>>(inst, err_part, nblk, t1, fixed, t_eval, n_eval, mlpw(), log, log_count, log_cap);
+    emit();
+    bfinalize_kernel<<<B, kBlock, 0, stream>>>(inst, ord_part, nblk, ring_cap, t1, r_ring);
+  }
+
+  int build_graphs() {
+    drop_graphs();
+    for (int which = 0; which < 2; ++which) {
+      hipGraph_t g = nullptr;
+      HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      for (int a = 0; a < (which ? 8 : 1); ++a) launch_attempt();
+      HIP_TRY(hipStreamEndCapture(stream, &g));
+      hipGraphExec_t ex = nullptr;
+      hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+      hipGraphDestroy(g);
+      if (e != hipSuccess) return kmt_fail(KMT_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
+      (which ? g8 : g1) = ex;
+    }
+    return KMT_OK;
+  }
+
+  int ensure_P() {
+    const size_t need = shared ? (size_t)16 * mpad * ldn : (size_t)2 * B * ldn;
+    if (P && P_elems >= need) return KMT_OK;
+    if (P) hipFree(P);
+    P = nullptr;
+    KMT_TRY(alloc(&P, need));
+    P_elems = need;
+    return KMT_OK;
+  }
+
+  int build_items() {
+    if (shared || items) return KMT_OK;
+    std::vector<GroupItem> it;
+    for (int b = 0; b < B; ++b)
+      for (long r = 0; r < Nb[b]; r += 8) it.push_back(GroupItem{b, (int)r});
+    n_items = (int)it.size();
+    KMT_TRY(alloc(&items, it.size()));
+    HIP_TRY(hipMemcpyAsync(items, it.data(), sizeof(GroupItem) * it.size(), hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    return KMT_OK;
+  }
+
+  int begin(const double *te, int ne, double t1_, int rcap) {
+    if (!A) return kmt_fail(KMT_ERR_STATE, "no adjacency: call kmt_batch_set_factor first");
+    if (rcap < 1) return kmt_fail(KMT_ERR_ARG, "ring_rows must be >= 1");
+    HIP_TRY(hipStreamSynchronize(stream));
+    drop_graphs();
+    KMT_TRY(ensure_P());
+    KMT_TRY(build_items());
+    if (t_eval) hipFree(t_eval);
+    if (ord_part) hipFree(ord_part);
+    if (r_ring) hipFree(r_ring);
+    t_eval = nullptr; ord_part = nullptr; r_ring = nullptr;
+    n_eval = ne;
+    ring_cap = rcap;
+    t1 = (float)t1_;
+    std::vector<float> tf(std::max(ne, 1));
+    for (int q = 0; q < ne; ++q) tf[q] = (float)te[q];
+    KMT_TRY(alloc(&t_eval, tf.size()));
+    HIP_TRY(hipMemcpyAsync(t_eval, tf.data(), sizeof(float) * tf.size(), hipMemcpyHostToDevice, stream));
+    KMT_TRY(alloc(&ord_part, (size_t)B * ring_cap * nblk * 2));
+    KMT_TRY(alloc(&r_ring, (size_t)B * ring_cap));
+    for (int b = 0; b < B; ++b) {
+      BInst &in = host_inst[b];
+      const float t0 = in.t, h0 = in.h, c1 = in.c1, ms = in.mscale;
+      const int n = in.n;
+      std::memset(&in, 0, sizeof(BInst));
+      in.t = t0; in.h = h0; in.c1 = c1; in.mscale = ms; in.n = n;
+      in.t_last = t0; in.t_prev = t0;
+      in.active = t0 < t1 ? 1 : 0;
+      in.rhs_evals = 1;
+      in.last_accept = 1;
+    }
+    HIP_TRY(hipMemcpyAsync(inst, host_inst.data(), sizeof(BInst) * B, hipMemcpyHostToDevice, stream));
+    // k1 = f(y) for every instance, then the first stage-2 vectors
+    dim3 g2((unsigned)nblk, (unsigned)B);
+    if (storage == KMT_ADJ_F16) bsincos_kernel<f16_t><<<g2, kBlock, 0, stream>>>(y, inst, ldn, cs[1], vt_out(), shared);
+    else bsincos_kernel<bf16_t><<<g2, kBlock, 0, stream>>>(y, inst, ldn, cs[1], vt_out(), shared);
+    cs_cur = cs[1];
+    coupling(0);
+    epilogue(KIND_RHS, bvecs(cs[1], nullptr));
+    emit();
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(stream));
+    begun = 1;
+    return KMT_OK;
+  }
+
+  int fetch() {
+    HIP_TRY(hipMemcpyAsync(host_inst.data(), inst, sizeof(BInst) * B, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    return KMT_OK;
+  }
+
+  void fill(kmt_batch_status *s) {
+    if (!s) return;
+    std::memset(s, 0, sizeof(*s));
+    s->t_min = 1e300;
+    s->t_max = -1e300;
+    for (const BInst &in : host_inst) {
+      s->accepted += in.accepted;
+      s->rejected += in.rejected;
+      s->attempts = std::max<long>(s->attempts, in.attempts);
+      s->rhs_evals += in.rhs_evals;
+      s->active += in.active;
+      s->errors += in.error;
+      s->rows_ready = std::max(s->rows_ready, in.ring_count);
+      s->t_min = std::min(s->t_min, (double)in.t);
+      s->t_max = std::max(s->t_max, (double)in.t);
+      s->osc_steps += (double)in.accepted * in.n;
+      s->proposer_accepts += in.proposer_accepts;
+      s->proposer_rejects += in.proposer_rejects;
+    }
+  }
+
+  int advance(int n, kmt_batch_status *s) {
+    if (!begun) return kmt_fail(KMT_ERR_STATE, "kmt_batch_advance before kmt_batch_begin");
+    KMT_TRY(fetch());
+    int any = 0;
+    for (const BInst &in : host_inst) any |= in.active;
+    if (any && n > 0) {
+      if (!g1) KMT_TRY(build_graphs());
+      for (; n >= 8; n -= 8) HIP_TRY(hipGraphLaunch(g8, stream));
+      for (; n > 0; --n) HIP_TRY(hipGraphLaunch(g1, stream));
+    }
+    KMT_TRY(fetch());
+    fill(s);
+    return KMT_OK;
+  }
+
+  int read_order(int b, int max_n, float *r, int *nout) {
+    if (b < 0 || b >= B) return kmt_fail(KMT_ERR_ARG, "instance out of range");
+    KMT_TRY(fetch());
+    const int ready = host_inst[b].ring_count;
+    const int n = std::min(std::max(max_n, 0), ready);
+    std::vector<float> buf(std::max(ready, 1));
+    if (ready) {
+      HIP_TRY(hipMemcpyAsync(buf.data(), r_ring + (size_t)b * ring_cap, sizeof(float) * ready, hipMemcpyDeviceToHost, stream));
+      HIP_TRY(hipStreamSynchronize(stream));
+    }
+    for (int q = 0; q < n; ++q) r[q] = buf[q];
+    if (n < ready) HIP_TRY(hipMemcpyAsync(r_ring + (size_t)b * ring_cap, buf.data() + n, sizeof(float) * (ready - n), hipMemcpyHostToDevice, stream));
+    const int left = ready - n;
+    HIP_TRY(hipMemcpyAsync(&inst[b].ring_count, &left, sizeof(int), hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    host_inst[b].ring_count = left;
+    if (nout) *nout = n;
+    return KMT_OK;
+  }
+
+  int get_instance(int b, float *theta, double *st) {
+    if (b < 0 || b >= B) return kmt_fail(KMT_ERR_ARG, "instance out of range");
+    KMT_TRY(fetch());
+    const BInst &in = host_inst[b];
+    if (theta) {
+      const float *src = (begun && in.commit_pending ? ynew : y) + (size_t)b * ldn;
+      HIP_TRY(hipMemcpyAsync(theta, src, sizeof(float) * Nb[b], hipMemcpyDeviceToHost, stream));
+      HIP_TRY(hipStreamSynchronize(stream));
+    }
+    if (st) {
+      st[0] = in.t; st[1] = in.h; st[2] = (double)in.accepted; st[3] = (double)in.rejected;
+      st[4] = in.active; st[5] = in.error; st[6] = (double)in.proposer_accepts; st[7] = (double)in.proposer_rejects;
+    }
+    return KMT_OK;
+  }
+
+  int time_attempts(int attempts, double *region_ms, double *coupling_ms, kmt_batch_status *s) {
+    if (!begun) return kmt_fail(KMT_ERR_STATE, "kmt_batch_time_attempts before kmt_batch_begin");
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> all(6 * attempts);
+    for (auto &p : all) {
+      HIP_TRY(hipEventCreate(&p.first));
+      HIP_TRY(hipEventCreate(&p.second));
+    }
+    hipEvent_t r0, r1;
+    HIP_TRY(hipEventCreate(&r0));
+    HIP_TRY(hipEventCreate(&r1));
+    auto work = all;
+    HIP_TRY(hipEventRecord(r0, stream));
+    for (int a = 0; a < attempts; ++a) launch_attempt(&work);
+    HIP_TRY(hipEventRecord(r1, stream));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventSynchronize(r1));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, r0, r1));
+    double tot = 0;
+    for (auto &p : all) {
+      float m = 0;
+      HIP_TRY(hipEventElapsedTime(&m, p.first, p.second));
+      tot += m;
+      hipEventDestroy(p.first);
+      hipEventDestroy(p.second);
+    }
+    hipEventDestroy(r0);
+    hipEventDestroy(r1);
+    if (region_ms) *region_ms = ms;
+    if (coupling_ms) *coupling_ms = tot / all.size();
+    KMT_TRY(fetch());
+    fill(s);
+    return KMT_OK;
+  }
+
+  // Tile autotuner of the shared (MFMA) product: n-tiles per wave {1, 2} x
+  // split-K {1, 2, 4, 8, 16}.  The split changes the order in which partial
+  // sums are added (results agree to FP32 rounding).
+  int autotune(int reps, int *best_nt, int *best_split, double *best_ms) {
+    if (!shared) return KMT_OK;
+    if (!A) return kmt_fail(KMT_ERR_STATE, "no adjacency");
+    KMT_TRY(ensure_P());
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    double best = 1e30;
+    int bnt = 1, bks = 4;
+    const int splits[5] = {1, 2, 4, 8, 16};
+    int idx = 0;
+    for (int nt = 1; nt <= 4; nt *= 2)
+      for (int si = 0; si < 5; ++si, ++idx) {
+        tile_nt = nt;
+        ksplit = splits[si];
+        gemm_t_any();
+        HIP_TRY(hipEventRecord(e0, stream));
+        for (int r = 0; r < reps; ++r) gemm_t_any();
+        HIP_TRY(hipEventRecord(e1, stream));
+        HIP_TRY(hipEventSynchronize(e1));
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        tune_ms[idx] = ms / reps;
+        if (ms < best) { best = ms; bnt = nt; bks = ksplit; }
+      }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    tile_nt = bnt;
+    ksplit = bks;
+    drop_graphs();
+    if (best_nt) *best_nt = bnt;
+    if (best_split) *best_split = bks;
+    if (best_ms) *best_ms = best / reps;
+    return KMT_OK;
+  }
+  void gemm_t_any() {
+    if (storage == KMT_ADJ_BF16) gemm_t<bf16_t>(); else gemm_t<f16_t>();
+  }
+
+  int set_proposer(const uint16_t *w, int nparams, int on) {
+    if (w) {
+      if (nparams != kMlpParams) return kmt_fail(KMT_ERR_ARG, "proposer needs %d parameters (got %d)", kMlpParams, nparams);
+      if (!mlp_w) KMT_TRY(alloc(&mlp_w, (size_t)kMlpParams));
+      HIP_TRY(hipMemcpyAsync(mlp_w, w, sizeof(uint16_t) * kMlpParams, hipMemcpyHostToDevice, stream));
+      HIP_TRY(hipStreamSynchronize(stream));
+    }
+    if (on && !mlp_w) return kmt_fail(KMT_ERR_STATE, "no proposer weights");
+    mlp_on = on;
+    drop_graphs();
+    return KMT_OK;
+  }
+
+  int set_log(int cap) {
+    drop_graphs();
+    if (log) { hipFree(log); log = nullptr; }
+    if (log_count) { hipFree(log_count); log_count = nullptr; }
+    log_cap = std::max(cap, 0);
+    if (log_cap) {
+      KMT_TRY(alloc(&log, (size_t)log_cap));
+      KMT_TRY(alloc(&log_count, 1));
+    }
+    return KMT_OK;
+  }
+
+  int read_log(int max_n, float *rec, int *nout) {
+    if (!log) return kmt_fail(KMT_ERR_STATE, "attempt log not enabled");
+    int cnt = 0;
+    HIP_TRY(hipMemcpyAsync(&cnt, log_count, sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    cnt = std::min(cnt, log_cap);
+    const int n = std::min(cnt, std::max(max_n, 0));
+    std::vector<LogRec> buf(std::max(n, 1));
+    if (n) {
+      HIP_TRY(hipMemcpyAsync(buf.data(), log, sizeof(LogRec) * n, hipMemcpyDeviceToHost, stream));
+      HIP_TRY(hipStreamSynchronize(stream));
+    }
+    for (int q = 0; q < n; ++q) {
+      float *o = rec + (size_t)q * 14;
+      for (int j = 0; j < kMaxFeat; ++j) o[j] = buf[q].x[j];
+      o[8] = buf[q].h_used; o[9] = buf[q].h_std; o[10] = buf[q].err_norm; o[11] = (float)buf[q].accepted;
+      o[12] = (float)buf[q].b; o[13] = (float)buf[q].after_accept;
+    }
+    const int zero = 0;
+    HIP_TRY(hipMemcpyAsync(log_count, &zero, sizeof(int), hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    if (nout) *nout = n;
+    return KMT_OK;
+  }
+};
+
+extern "C" {
+
+#define B_CHECK \
+  if (!bp) return kmt_fail(KMT_ERR_ARG, "batch is NULL");
+
+int kmt_batch_create(kmt_batch **out, int device, int B, const long *N, double atol, double rtol, int shared,
+                     int storage, int fixed_step) {
+  if (!out || !N || B < 1) return kmt_fail(KMT_ERR_ARG, "bad batch arguments");
+  *out = nullptr;
+  for (int b = 0; b < B; ++b)
+    if (N[b] < 1) return kmt_fail(KMT_ERR_ARG, "instance %d: N must be >= 1", b);
+  if (storage != KMT_ADJ_NATIVE && storage != KMT_ADJ_BF16 && storage != KMT_ADJ_F16)
+    return kmt_fail(KMT_ERR_ARG, "unknown storage %d", storage);
+  int ndev = 0;
+  HIP_TRY(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return kmt_fail(KMT_ERR_ARG, "device %d not present", device);
+  kmt_batch *bp = new kmt_batch();
+  int rc = bp->create(device, B, N, atol, rtol, shared, storage, fixed_step);
+  if (rc != KMT_OK) {
+    delete bp;
+    return rc;
+  }
+  *out = bp;
+  return KMT_OK;
+}
+int kmt_batch_destroy(kmt_batch *bp) {
+  delete bp;
+  return KMT_OK;
+}
+int kmt_batch_set_instance(kmt_batch *bp, int b, const float *omega, const float *theta0, double c1, double t0,
+                           double h0) {
+  B_CHECK;
+  if (!omega || !theta0) return kmt_fail(KMT_ERR_ARG, "NULL omega or theta0");
+  return bp->set_instance(b, omega, theta0, c1, t0, h0);
+}
+int kmt_batch_set_factor(kmt_batch *bp, int b, const float *X, const float *w, int K) {
+  B_CHECK;
+  if (K < 0 || (K > 0 && (!X || !w))) return kmt_fail(KMT_ERR_ARG, "bad factor");
+  return bp->set_factor(b, X, w, K);
+}
+int kmt_batch_get_adjacency_rows(kmt_batch *bp, int b, long i0, long i1, float *out) {
+  B_CHECK;
+  return bp->get_adjacency_rows(b, i0, i1, out);
+}
+int kmt_batch_begin(kmt_batch *bp, const double *t_eval, int n_eval, double t1, int ring_rows) {
+  B_CHECK;
+  if (n_eval > 0 && !t_eval) return kmt_fail(KMT_ERR_ARG, "t_eval is NULL");
+  return bp->begin(t_eval, n_eval, t1, ring_rows);
+}
+int kmt_batch_advance(kmt_batch *bp, int max_attempts, kmt_batch_status *st) {
+  B_CHECK;
+  return bp->advance(max_attempts, st);
+}
+int kmt_batch_read_order(kmt_batch *bp, int b, int max_n, float *r, int *n) {
+  B_CHECK;
+  return bp->read_order(b, max_n, r, n);
+}
+int kmt_batch_get_instance(kmt_batch *bp, int b, float *theta, double *state8) {
+  B_CHECK;
+  return bp->get_instance(b, theta, state8);
+}
+int kmt_batch_autotune(kmt_batch *bp, int reps, int *n_tiles, int *split_k, double *best_ms) {
+  B_CHECK;
+  return bp->autotune(reps, n_tiles, split_k, best_ms);
+}
+int kmt_batch_time_attempts(kmt_batch *bp, int attempts, double *region_ms, double *coupling_ms,
+                            kmt_batch_status *st) {
+  B_CHECK;
+  if (attempts < 1) return kmt_fail(KMT_ERR_ARG, "attempts < 1");
+  return bp->time_attempts(attempts, region_ms, coupling_ms, st);
+}
+int kmt_batch_tune_table(kmt_batch *bp, double *ms15) {
+  B_CHECK;
+  if (ms15) for (int q = 0; q < 15; ++q) ms15[q] = bp->tune_ms[q];
+  return KMT_OK;
+}
+int kmt_batch_set_vector_split(kmt_batch *bp, int hi_lo) {
+  B_CHECK;
+  bp->lo = hi_lo ? 1 : 0;
+  bp->drop_graphs();
+  return KMT_OK;
+}
+int kmt_batch_set_proposer(kmt_batch *bp, const uint16_t *weights_f16, int n_params, int enabled) {
+  B_CHECK;
+  return bp->set_proposer(weights_f16, n_params, enabled);
+}
+int kmt_batch_set_attempt_log(kmt_batch *bp, int capacity) {
+  B_CHECK;
+  return bp->set_log(capacity);
+}
+int kmt_batch_read_attempt_log(kmt_batch *bp, int max_n, float *records, int *n) {
+  B_CHECK;
+  if (!records) return kmt_fail(KMT_ERR_ARG, "records is NULL");
+  return bp->read_log(max_n, records, n);
+}
+int kmt_proposer_param_count(void) { return kMlpParams; }
+
+}  // extern "C"

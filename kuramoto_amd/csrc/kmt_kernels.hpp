@@ -749,43 +749,4 @@ __global__ __launch_bounds__(kBlock) void read_rows_kernel(const S *A, long lda,
     dst[(size_t)r * ncols + j] = load_value<R, S>(A[(size_t)r * lda + j], scale);
 }
 
-// ---------------------------------------------------------------------
-// Bandwidth probes: a plain 16-byte copy and a read-only stream of known
-// size (the measured ceiling next to the nominal 8 TB/s, and the
-// calibration kernel for the HBM byte counters).
-// ---------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void copy_probe_kernel(const u32x4 *src, u32x4 *dst, size_t n16) {
-  const size_t stride = (size_t)gridDim.x * kBlock;
-  size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
-  for (; i + 3 * stride < n16; i += 4 * stride) {
-    const u32x4 a = __builtin_nontemporal_load(src + i);
-    const u32x4 b = __builtin_nontemporal_load(src + i + stride);
-    const u32x4 c = __builtin_nontemporal_load(src + i + 2 * stride);
-    const u32x4 d = __builtin_nontemporal_load(src + i + 3 * stride);
-    dst[i] = a;
-    dst[i + stride] = b;
-    dst[i + 2 * stride] = c;
-    dst[i + 3 * stride] = d;
-  }
-  for (; i < n16; i += stride) dst[i] = __builtin_nontemporal_load(src + i);
-}
-
-__global__ __launch_bounds__(kBlock) void read_probe_kernel(const u32x4 *src, size_t n16, unsigned int *sink) {
-  const size_t stride = (size_t)gridDim.x * kBlock;
-  size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
-  unsigned int acc = 0;
-  for (; i + 3 * stride < n16; i += 4 * stride) {
-    const u32x4 a = __builtin_nontemporal_load(src + i);
-    const u32x4 b = __builtin_nontemporal_load(src + i + stride);
-    const u32x4 c = __builtin_nontemporal_load(src + i + 2 * stride);
-    const u32x4 d = __builtin_nontemporal_load(src + i + 3 * stride);
-    acc ^= a.x ^ a.y ^ a.z ^ a.w ^ b.x ^ b.y ^ b.z ^ b.w ^ c.x ^ c.y ^ c.z ^ c.w ^ d.x ^ d.y ^ d.z ^ d.w;
-  }
-  for (; i < n16; i += stride) {
-    const u32x4 v = __builtin_nontemporal_load(src + i);
-    acc ^= v.x ^ v.y ^ v.z ^ v.w;
-  }
-  if (acc == 0x9E3779B9u) sink[0] = acc;  // keeps the loads alive; practically never taken
-}
-
 }  // namespace kmt

@@ -1,0 +1,143 @@
+"""Train the learned step-size proposer of the batched integrator.
+
+1. Log collection: the batched integrator (grouped mode, proposer off, attempt
+   log on) integrates instances shaped like the three plain configurations:
+     c1  N=5000, K=5 Gaussian volcano, Cauchy f=1, theta0=0, c=3, t=100, dt=1
+     c2  N=10000 (the FP64 N=50000 case reduced to fit the logging budget),
+         normal frequencies, theta0 = pi(2u-1), c=3, t=100, dt=1
+     c3  N=20000 (the N=200000 low-rank case reduced), BF16 adjacency,
+         Cauchy, theta0=0, c=3, t=20, dt=0.5
+   all at atol=1e-6, several seeds each.
+2. Samples: every attempt whose step came from the proposal made after an
+   accepted step; features = the 8 proposer inputs logged by the device
+   controller (last 3 log h relative to the proposal, last 3 log error
+   ratios, r, dr/dt); target = log(h_opt / h_proposed) = (1/5) log(theta /
+   err), the step that would have produced the error level theta (DP45 error
+   ~ h^5), clipped to the proposer's range [-0.7, 0.7].
+3. Training on the GPU with the library's own kernels (kmt_mlp_train: Adam,
+   FP32 master weights, FP16 copy).
+4. Selection of theta on a held-out batch shaped like the uneven-size batched
+   configuration: the theta whose network needs the fewest RHS evaluations
+   while every instance's r(t) stays within 1e-3 of the run without it.
+Writes kuramoto_amd/proposer_weights.npz (FP16 weights + metadata).
+
+    python -m kuramoto_amd.train_proposer [--quick]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+
+import kuramoto_amd as km  # noqa: E402
+
+OUT = os.path.join(HERE, "proposer_weights.npz")
+
+
+def _grid(t1, dt):
+    dt = np.float32(dt)
+    return (dt * np.arange(1, int(np.float32(t1) / dt) + 1)).astype(np.float64)
+
+
+def make_batch(specs, storage="native", log_cap=0, weights=None):
+    """specs: list of dicts N, seed, normal, init, c."""
+    bt = km.Batch([s["N"] for s in specs], atol=1e-6, shared=False, storage=storage)
+    for b, s in enumerate(specs):
+        th, om = km.draws(km.F32, s["N"], s["seed"], normal=s["normal"], init=s["init"])
+        X = km.factor(s["N"], 5, s["seed"], gaussian=True).astype(np.float32)
+        w = km.factor_weights(km.F32, s["N"], 5, gaussian=True)
+        bt.set_instance(b, om, th, s["c"], 0.0, s.get("h0", 0.01))
+        bt.set_factor(b, X, w)
+    if log_cap:
+        bt.set_attempt_log(log_cap)
+    if weights is not None:
+        bt.set_proposer(weights, True)
+    return bt
+
+
+def collect_logs(quick):
+    seeds = range(1, 3 if quick else 7)
+    groups = [
+        ([dict(N=5000, seed=s, normal=False, init=0.0, c=3.0, h0=0.01) for s in seeds], "native", 100.0, 1.0),
+        ([dict(N=10000, seed=s + 1, normal=True, init=6.283185307, c=3.0, h0=0.01) for s in seeds], "native", 100.0,
+         1.0),
+        ([dict(N=20000, seed=s + 2, normal=False, init=0.0, c=3.0, h0=0.005) for s in seeds], "bf16", 20.0, 0.5),
+    ]
+    logs = []
+    for specs, storage, t1, dt in groups:
+        t0 = time.time()
+        with make_batch(specs, storage, log_cap=400000) as bt:
+            res = bt.run(_grid(t1, dt), t1, chunk=64)
+            rec = bt.read_attempt_log(400000)
+        st = res["status"]
+        print(f"logs: {len(specs)} x N={specs[0]['N']} t1={t1}: {len(rec)} attempts, {st.accepted} accepted, "
+              f"{st.rejected} rejected, {time.time() - t0:.1f} s", flush=True)
+        logs.append(rec)
+    return np.concatenate(logs)
+
+
+def dataset(rec, theta):
+    sel = rec[:, 13] > 0.5
+    X = rec[sel, :8]
+    err = np.maximum(rec[sel, 10], 1e-12)
+    y = np.clip(0.2 * np.log(theta / err), -0.7, 0.7)
+    ok = np.all(np.isfinite(X), axis=1) & np.isfinite(y)
+    return X[ok].astype(np.float32), y[ok].astype(np.float32)
+
+
+def evaluate_specs(quick):
+    Ns = [2000, 4000, 6000, 3000] if quick else [5000, 10000, 15000, 20000, 25000, 30000, 35000, 40000]
+    return [dict(N=n, seed=200 + b, normal=b % 2 == 0, init=6.283185307, c=3.0) for b, n in enumerate(Ns)]
+
+
+def run_eval(specs, weights):
+    """Uneven-size batch (the C5 shape, scaled down in time): RHS
+    evaluations, rejections, r series."""
+    t1 = 20.0
+    with make_batch(specs, "bf16", weights=weights) as bt:
+        res = bt.run(_grid(t1, 1.0), t1, chunk=64)
+    st = res["status"]
+    return dict(rhs=st.rhs_evals, rejected=st.rejected, accepted=st.accepted, r=res["r"],
+                proposer=(st.proposer_accepts, st.proposer_rejects))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--quick", action="store_true")
+    ap.add_argument("--epochs", type=int, default=60)
+    args = ap.parse_args()
+    rec = collect_logs(args.quick)
+    specs = evaluate_specs(args.quick)
+    std = run_eval(specs, None)
+    print(f"standard controller on the held-out batch: rhs={std['rhs']} rejected={std['rejected']} "
+          f"accepted={std['accepted']}", flush=True)
+    cands = []
+    for theta in (0.6, 0.75, 0.9):
+        X, y = dataset(rec, theta)
+        perm = np.random.default_rng(0).permutation(len(y))
+        X, y = X[perm], y[perm]
+        with km.Mlp(seed=11) as m:
+            hist = m.train(X, y, epochs=args.epochs, batch=1024, lr=1e-3)
+            w32, w16 = m.weights()
+        res = run_eval(specs, w16)
+        dr = max(float(np.max(np.abs(a - b))) for a, b in zip(std["r"], res["r"]))
+        row = dict(theta=theta, samples=int(len(y)), loss_first=float(hist[0]), loss_last=float(hist[-1]),
+                   rhs_standard=int(std["rhs"]), rhs_learned=int(res["rhs"]), rej_standard=int(std["rejected"]),
+                   rej_learned=int(res["rejected"]), acc_standard=int(std["accepted"]),
+                   acc_learned=int(res["accepted"]), proposer=list(map(int, res["proposer"])), max_dr=dr)
+        print(json.dumps(row), flush=True)
+        cands.append((row, w16, w32))
+    ok = [c for c in cands if c[0]["max_dr"] < 1e-3]
+    best = min(ok or cands, key=lambda c: c[0]["rhs_learned"])
+    row, w16, w32 = best
+    np.savez(OUT, w16=w16, w32=w32, theta=row["theta"], report=json.dumps([c[0] for c in cands]))
+    print("wrote", OUT, "theta", row["theta"], flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -4,6 +4,7 @@
  * CPU-baseline leg of bench.py.  Nothing in the product links this library.
  */
 #include "oracle.hpp"
+#include "mlp_ref.hpp"
 
 using namespace orc;
 
@@ -194,3 +195,10 @@ int orc_num_threads(void) {
 }
 
 }  // extern "C"
+
+extern "C" {
+double orc_mlp_loss_grad(const double *w, const double *X, const double *y, int n, double *grad, double *out) {
+  return orc::mlp_loss_grad(w, X, y, n, grad, out);
+}
+int orc_mlp_param_count(void) { return orc::MP; }
+}

@@ -171,3 +171,17 @@ def integrate(adj: Adj, omega, c1, theta0, t0, h0, t1, atol, rtol=0.0, fixed=Fal
 
 def num_threads():
     return load().orc_num_threads()
+
+
+def mlp_loss_grad(w, X, y):
+    """FP64 proposer network: (loss, grad, out)."""
+    w = np.ascontiguousarray(w, np.float64)
+    X = np.ascontiguousarray(X, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    n = X.shape[0]
+    g = np.zeros(load().orc_mlp_param_count(), np.float64)
+    out = np.zeros(n, np.float64)
+    fn = load().orc_mlp_loss_grad
+    fn.restype = ctypes.c_double
+    loss = fn(_p(w), _p(X), _p(y), n, _p(g), _p(out))
+    return loss, g, out

@@ -253,3 +253,35 @@ def test_resume_bit_exact(orc):
     rest = orc.integrate(orc.Adj(0, M=M), om, 3.0, part["y"], part["t"], part["h"], 2.0, 1e-9)
     assert rest["t"] == full["t"] and np.array_equal(rest["y"], full["y"])
     assert part["accepted"] + rest["accepted"] == full["accepted"]
+
+
+# ------------------------------------------------------- proposer network
+def test_mlp_oracle_gradient_finite_differences(orc):
+    """Back-propagated gradient of the FP64 network against central finite
+    differences on a sample of parameters of every layer."""
+    rng = np.random.default_rng(0)
+    P = 4801
+    w = rng.standard_normal(P) * 0.2
+    X = rng.standard_normal((40, 8))
+    y = rng.standard_normal(40) * 0.3
+    loss, g, _ = orc.mlp_loss_grad(w, X, y)
+    idx = [0, 7, 300, 511, 512, 575, 576, 1000, 4671, 4672, 4735, 4736, 4799, 4800]
+    for p in idx:
+        e = np.zeros(P)
+        e[p] = 1e-6
+        fd = (orc.mlp_loss_grad(w + e, X, y)[0] - orc.mlp_loss_grad(w - e, X, y)[0]) / 2e-6
+        assert abs(fd - g[p]) < 1e-7 + 1e-5 * abs(fd), (p, fd, g[p])
+
+
+def test_mlp_oracle_zero_network(orc):
+    """All-zero weights except b3: out = b3, dL/db3 = 2 mean(b3 - y), the
+    output-layer gradient is zero (h2 = 0) and nothing flows back."""
+    w = np.zeros(4801)
+    w[-1] = 0.25
+    X = np.random.default_rng(1).standard_normal((10, 8))
+    y = np.linspace(-1, 1, 10)
+    loss, g, out = orc.mlp_loss_grad(w, X, y)
+    assert np.allclose(out, 0.25)
+    assert abs(loss - np.mean((0.25 - y) ** 2)) < 1e-15
+    assert abs(g[-1] - 2 * np.mean(0.25 - y)) < 1e-15
+    assert np.all(g[:-1] == 0)
