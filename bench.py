@@ -41,6 +41,25 @@ CONFIGS = {
     # the reference-defaults FP32 case (MALL-resident A)
     "fp32-n5000": dict(N=5000, K=5, seed=1, prec="f32", normal=False, init=0.0, c1=3.0, atol=1e-6, t1=100.0,
                        dt=1.0, storage="native"),
+    # -A, the low-rank factor only (N x K), coupled through the factor
+    "fp32-n200000-lowrank": dict(N=200000, K=5, seed=3, prec="f32", normal=False, init=0.0, c1=3.0, atol=1e-6,
+                                 t1=20.0, dt=0.5, storage="factored"),
+    # configs[2]'s check run: the same A stored explicitly in FP16 (80 GB)
+    "fp32-n200000-f16": dict(N=200000, K=5, seed=3, prec="f32", normal=False, init=0.0, c1=3.0, atol=1e-6,
+                             t1=20.0, dt=0.5, storage="f16"),
+}
+
+# batched configurations (FP32 state, 16-bit adjacency)
+BATCH_CONFIGS = {
+    # B=64 sharing one N=20000 BF16 adjacency, c_b = linspace(0.5, 4, 64),
+    # Cauchy frequencies, theta0 = pi(2u-1), seeds 1..64, t=50
+    "batch64-n20000-bf16": dict(B=64, Ns=[20000] * 64, shared=True, seeds=list(range(1, 65)),
+                                cs=list(np.linspace(0.5, 4.0, 64)), normal=[False] * 64, t1=50.0, dt=1.0),
+    # B=48, N uniform in {5000..40000}, own adjacency each (seeds
+    # 100..147), normal/Cauchy alternating, c=3, t=100, with the learned
+    # step proposer against the same batch without it
+    "batch48-uneven-bf16": dict(B=48, Ns=None, shared=False, seeds=list(range(100, 148)), cs=[3.0] * 48,
+                                normal=[b % 2 == 0 for b in range(48)], t1=100.0, dt=1.0),
 }
 
 HBM_PEAK = 8.0e12
@@ -74,7 +93,10 @@ def gpu_run(cfg, steps, warmup):
     it.set_omega(omega)
     it.set_factor(X, w)
     t0 = time.time()
-    it.build_adjacency(cfg["storage"])
+    if cfg["storage"] == "factored":
+        it.use_factored(True)
+    else:
+        it.build_adjacency(cfg["storage"])
     gen_s = time.time() - t0
     it.set_state(theta0, 0.0, float(dt / dtype(100)))
     it.begin(te, cfg["t1"], keep_rows=False, ring_rows=4096)
@@ -103,18 +125,127 @@ def cpu_baseline(cfg, max_steps):
     theta0, omega = orc.draws(N, cfg["seed"], cfg["normal"], 1.0, cfg["init"], dtype=dtype)
     X = orc.gaussian_factor(N, K, cfg["seed"]).astype(dtype)
     w = orc.factor_weights(N, K, True, dtype=dtype)
-    M = orc.adj_rows(X, w, 0, N)
+    # N^2 does not fit the host for the N=200000 cases: the oracle runs the
+    # reference's own -A path (coupling through the factor) there
+    lowrank = cfg["storage"] in ("factored", "f16")
+    adj = orc.Adj(2, X=X, w=w) if lowrank else orc.Adj(0, M=orc.adj_rows(X, w, 0, N))
     dt = dtype(cfg["dt"])
     te = (dt * np.arange(1, int(dtype(cfg["t1"]) / dt) + 1)).astype(dtype)
     t0 = time.time()
-    res = orc.integrate(orc.Adj(0, M=M), omega, cfg["c1"], theta0, 0.0, float(dt / dtype(100)), cfg["t1"],
+    res = orc.integrate(adj, omega, cfg["c1"], theta0, 0.0, float(dt / dtype(100)), cfg["t1"],
                         cfg["atol"], t_eval=te, keep_rows=False, max_steps=max_steps, dtype=dtype)
     secs = time.time() - t0
-    del M
+    del adj
     return dict(value=N * res["accepted"] / secs, unit="osc*steps/s", cores=orc.num_threads(), kind="port",
-                sample=f"oracle {np.dtype(dtype).name} dense, N={N}, first {res['accepted']} accepted steps "
+                sample=f"oracle {np.dtype(dtype).name} {'low-rank (-A)' if lowrank else 'dense'}, N={N}, "
+                       f"first {res['accepted']} accepted steps "
                        f"({res['rejected']} rejected, {res['rhs_evals']} RHS evals) in {secs:.1f} s, "
                        f"OpenMP threads={orc.num_threads()}")
+
+
+def uneven_sizes(B, seed=147):
+    """N_b drawn uniformly from {5000, 10000, ..., 40000} (Philox-free, fixed)."""
+    rng = np.random.default_rng(seed)
+    return [int(5000 * rng.integers(1, 9)) for _ in range(B)]
+
+
+def batch_run(name, steps, warmup, proposer):
+    import kuramoto_amd as km
+
+    cfg = BATCH_CONFIGS[name]
+    B = cfg["B"]
+    Ns = cfg["Ns"] or uneven_sizes(B)
+    bt = km.Batch(Ns, atol=1e-6, shared=cfg["shared"], storage="bf16")
+    if cfg["shared"]:
+        X = km.factor(Ns[0], 5, 1, gaussian=True).astype(np.float32)
+        bt.set_factor(-1, X, km.factor_weights(km.F32, Ns[0], 5, gaussian=True))
+    for b in range(B):
+        th, om = km.draws(km.F32, Ns[b], cfg["seeds"][b], normal=cfg["normal"][b], init=6.283185307)
+        bt.set_instance(b, om, th, float(cfg["cs"][b]), 0.0, 0.01)
+        if not cfg["shared"]:
+            X = km.factor(Ns[b], 5, cfg["seeds"][b], gaussian=True).astype(np.float32)
+            bt.set_factor(b, X, km.factor_weights(km.F32, Ns[b], 5, gaussian=True))
+    tune = bt.autotune(3) if cfg["shared"] else None
+    if proposer:
+        w = np.load(os.path.join(ROOT, "kuramoto_amd", "proposer_weights.npz"))["w16"]
+        bt.set_proposer(w, True)
+    dt = np.float32(cfg["dt"])
+    te = (dt * np.arange(1, int(np.float32(cfg["t1"]) / dt) + 1)).astype(np.float64)
+    bt.begin(te, cfg["t1"], ring_rows=1024)
+    if warmup:
+        bt.advance(warmup)
+    st0 = bt.advance(0)
+    region_ms, coup_ms, st = bt.time_attempts(steps)
+    bt.close()
+    return dict(Ns=Ns, region_ms=region_ms, coupling_ms=coup_ms, st0=st0, st=st, tune=tune)
+
+
+def plain_attempt_ms(N, steps=10):
+    """One plain FP32 dense run at size N (FP32 adjacency): ms per attempt."""
+    import kuramoto_amd as km
+
+    X = km.factor(N, 5, 1, gaussian=True).astype(np.float32)
+    th, om = km.draws(km.F32, N, 1, normal=False, init=6.283185307)
+    with km.Integrator(N, km.F32, atol=1e-6, c1=3.0) as it:
+        it.set_omega(om)
+        it.set_factor(X, km.factor_weights(km.F32, N, 5, gaussian=True))
+        it.build_adjacency()
+        it.set_state(th, 0.0, 0.01)
+        it.begin(np.arange(1, 51, dtype=np.float64), 50.0, keep_rows=False, ring_rows=64)
+        it.advance(2)
+        region_ms, _, _ = it.time_attempts(steps)
+    return region_ms / steps
+
+
+def batch_main(args):
+    cfg = BATCH_CONFIGS[args.config]
+    r = batch_run(args.config, args.steps, args.warmup, proposer=not cfg["shared"])
+    st, st0 = r["st"], r["st0"]
+    secs = r["region_ms"] / 1e3
+    osc_steps = st.osc_steps - st0.osc_steps
+    Ns = r["Ns"]
+    if cfg["shared"]:
+        N = Ns[0]
+        mpad = 128
+        alg = 2 * N * N + 2 * 2 * mpad * N  # BF16 A once + hi/lo V^T once
+        seq_ms = plain_attempt_ms(N)
+        inst_steps = st.accepted - st0.accepted
+        per_attempt_batched = r["region_ms"] / args.steps
+        extra = {"sequential_plain_fp32_ms_per_attempt_x64": 64 * seq_ms,
+                 "batched_ms_per_attempt": per_attempt_batched,
+                 "speedup_vs_64_sequential_plain": 64 * seq_ms / per_attempt_batched,
+                 "tile": {"n_tiles_per_wave": r["tune"][0], "split_k": r["tune"][1]}}
+    else:
+        alg = sum(2 * n * n + 3 * 4 * n for n in Ns)
+        base = batch_run(args.config, args.steps, args.warmup, proposer=False)
+        b_steps = base["st"].osc_steps - base["st0"].osc_steps
+        extra = {"standard_controller": {"osc_steps_per_s": b_steps / (base["region_ms"] / 1e3),
+                                         "rhs_evals": base["st"].rhs_evals - base["st0"].rhs_evals,
+                                         "rejected": base["st"].rejected - base["st0"].rejected,
+                                         "accepted": base["st"].accepted - base["st0"].accepted},
+                 "learned_proposer": {"rhs_evals": st.rhs_evals - st0.rhs_evals,
+                                      "rejected": st.rejected - st0.rejected,
+                                      "accepted": st.accepted - st0.accepted,
+                                      "proposer_accepts": st.proposer_accepts - st0.proposer_accepts,
+                                      "proposer_rejects": st.proposer_rejects - st0.proposer_rejects},
+                 "sum_N": int(sum(Ns))}
+    achieved = alg / (r["coupling_ms"] / 1e3)
+    out = {
+        "metric": "osc*steps/s (sum over instances of N_b x accepted steps per second)",
+        "value": osc_steps / secs,
+        "unit": "osc*steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": r["region_ms"] / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 (bf16 adjacency)",
+        "data": "synthetic (Philox Gaussian factors, glibc-rand frequencies/phases)",
+        "config": {"workload": args.config, "B": cfg["B"], "instance_steps_in_region": st.accepted - st0.accepted,
+                   **extra},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "kernel": "bgemm_kernel (MFMA)" if cfg["shared"] else "bgemv_kernel (grouped)",
+                     "bytes_per_launch": alg, "avg_launch_ms": r["coupling_ms"]},
+    }
+    print(json.dumps(out), flush=True)
 
 
 def main():
@@ -122,20 +253,27 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="fp64-n50000", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="fp64-n50000", choices=sorted(CONFIGS) + sorted(BATCH_CONFIGS))
     ap.add_argument("--cpu-steps", type=int, default=20, help="accepted steps of the CPU baseline sample (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (see DESIGN.md, "
               "'How the path would shard')", file=sys.stderr)
         sys.exit(2)
+    if args.config in BATCH_CONFIGS:
+        batch_main(args)
+        return
     cfg = CONFIGS[args.config]
     r = gpu_run(cfg, args.steps, args.warmup)
     N = r["N"]
     s_y = np.dtype(r["dtype"]).itemsize
     accepted = r["st"].accepted - r["st0"].accepted
     secs = r["region_ms"] / 1e3
-    alg_bytes = r["elem_bytes"] * N * N + N * (2 * s_y + s_y)
+    if cfg["storage"] == "factored":
+        # per RHS: X read twice (N K s_y each) + theta-derived vectors in, k out, omega
+        alg_bytes = 2 * N * cfg["K"] * s_y + N * (2 * s_y + s_y)
+    else:
+        alg_bytes = r["elem_bytes"] * N * N + N * (2 * s_y + s_y)
     achieved = alg_bytes / (r["stage_ms"] / 1e3)
     out = {
         "metric": "osc*steps/s (N x accepted DP45 steps per second)",
@@ -150,14 +288,17 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if s_y == 8 else "f32",
         "data": "synthetic (Philox Gaussian factor, glibc-rand frequencies/phases, as the CLI generates)",
-        "config": {"workload": f"{args.config}: N={N}, K={cfg['K']}, dense adjacency {cfg['storage']} "
-                               f"({r['elem_bytes']} B/elem), atol={cfg['atol']}, c={cfg['c1']}",
+        "config": {"workload": f"{args.config}: N={N}, K={cfg['K']}, "
+                               + (f"low-rank factor only (-A)" if cfg["storage"] == "factored" else
+                                  f"dense adjacency {cfg['storage']} ({r['elem_bytes']} B/elem)")
+                               + f", atol={cfg['atol']}, c={cfg['c1']}",
                    "N": N, "accepted_in_region": accepted,
                    "rejected_in_region": r["st"].rejected - r["st0"].rejected,
                    "t_after": r["st"].t, "adjacency_build_s": round(r["gen_s"], 3)},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": profiled_traffic(args.config),
-                     "kernel": "stage_kernel (fused gemv-pair + DP45 stage epilogue)",
+                     "kernel": ("lowrank partial+reduce+expand (one RHS)" if cfg["storage"] == "factored"
+                                else "stage_kernel (fused gemv-pair + DP45 stage epilogue)"),
                      "bytes_per_launch": alg_bytes, "avg_launch_ms": r["stage_ms"],
                      "variant": {"rows_per_workgroup": r["rows"], "nontemporal_loads": r["nt"]},
                      "measured_read_stream_gbps": r["read_gbps"], "measured_copy_gbps": r["copy_gbps"],
