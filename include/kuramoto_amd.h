@@ -226,9 +226,10 @@ int kmt_batch_read_order(kmt_batch *batch, int b, int max_n, float *r, int *n);
 /* phases at the last accepted step; state8 = {t, h, accepted, rejected,
  * active, error, proposer accepts, proposer rejects} (either may be NULL) */
 int kmt_batch_get_instance(kmt_batch *batch, int b, float *theta, double *state8);
-/* tile autotuner of the shared MFMA product: n-tiles per wave {1, 2, 4} x
- * split-K {1, 2, 4, 8, 16}, `reps` launches each; keeps the fastest.
- * kmt_batch_tune_table returns the 15 average times (ms), n-tile major. */
+/* tile autotuner of the shared MFMA product: rows of A per workgroup
+ * (n_tiles x 128, n_tiles in {1, 2, 4}) x split-K {1, 2, 4, 8, 16}, `reps`
+ * launches each; keeps the fastest.  kmt_batch_tune_table returns the 15
+ * average times (ms), n_tiles major. */
 int kmt_batch_autotune(kmt_batch *batch, int reps, int *n_tiles, int *split_k, double *best_ms);
 int kmt_batch_tune_table(kmt_batch *batch, double *ms15);
 /* HIP-event timing of `attempts` attempts; coupling_ms = average of the

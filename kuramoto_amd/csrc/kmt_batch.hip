@@ -96,16 +96,32 @@ __global__ __launch_bounds__(kBlock) void bsincos_kernel(const float *y, const B
 
 // ---------------------------------------------------------------------
 // Shared adjacency: Z^T (Mpad x N) = V^T (Mpad x K) . A^T (K x N) on MFMA.
-// Workgroup = 4 waves; wave w owns oscillators n0 + [0, 32 NT) and all MT
-// 32-row tiles of V^T; K is split over gridDim.y slabs.  V^T chunks
-// (MT*32 x 64 hi and lo) are staged in LDS and shared by the 4 waves; A
-// fragments (8 consecutive 16-bit values of one row per lane) come straight
-// from HBM.  Operand layouts (verified on gfx950): A-operand lane l holds
-// row l%32, k = 8(l/32)..+7; B-operand lane l holds column l%32, the same
-// k; D lane l, item v holds row 8(v/4) + 4(l/32) + v%4, column l%32.
+//
+// A is stored in MFMA fragment order: 32-row x 64-column tiles, each 4 KB
+// (bf16/f16) and laid out as 4 k-steps x 64 lanes x 8 values, so that one
+// wave's fragment load for a k-step is 1 KB of consecutive bytes and a wave
+// streams one contiguous run of HBM along its rows (tiles of one 32-row
+// band are consecutive in k).  Element (i, j) sits at tiled_offset(i, j).
+//
+// Workgroup = WAVES waves; wave w owns the NT 32-row bands starting at band
+// (rb WAVES + w) NT and all MT 32-row tiles of V^T.  K is split into ksplit
+// slabs; blockIdx.x = rb ksplit + slab, so with ksplit a multiple or
+// divisor of 8 the workgroups of one XCD (dispatched round-robin, block % 8)
+// work on the same slab(s) and share its V^T rows in their L2.  V^T chunks
+// (MT*32 x 64, hi and lo) are double-buffered in LDS: the next chunk's A
+// fragments and V^T pieces are loaded before the current chunk's MFMAs.
+// Operand layouts (verified on gfx950): A-operand lane l holds row l%32,
+// k = 8(l/32)..+7; B-operand lane l holds column l%32, the same k; D lane
+// l, item v holds row 8(v/4) + 4(l/32) + v%4, column l%32.
 // ---------------------------------------------------------------------
-constexpr int kKC = 64;        // K per LDS chunk
+constexpr int kKC = 64;        // K per chunk (one tile column)
 constexpr int kLdsRow = 72;    // padded LDS row (64 + 8 halves)
+constexpr long kTile = 32L * kKC;  // values per tile
+
+__host__ __device__ __forceinline__ size_t tiled_offset(long i, long j, long nkc) {
+  const long lane = (i & 31) + 32 * ((j & 15) >> 3);
+  return ((((size_t)(i >> 5) * nkc + (j >> 6)) * 4 + ((j & 63) >> 4)) * 64 + lane) * 8 + (j & 7);
+}
 
 template <class T> __device__ __forceinline__ f32x16_t mfma32(const u32x4 a, const u32x4 b, f32x16_t c);
 template <> __device__ __forceinline__ f32x16_t mfma32<bf16_t>(const u32x4 a, const u32x4 b, f32x16_t c) {
@@ -117,15 +133,25 @@ template <> __device__ __forceinline__ f32x16_t mfma32<f16_t>(const u32x4 a, con
                                                 0, 0, 0);
 }
 
-template <class T, int MT, int NT, bool LO>
-__global__ __launch_bounds__(kBlock) void bgemm_kernel(const T *A, long lda, const uint16_t *vthi, const uint16_t *vtlo,
-                                                       long ldk, long kslab, float *P, long ldn, int mpad) {
-  __shared__ uint16_t sh_hi[MT * 32 * kLdsRow];
-  __shared__ uint16_t sh_lo[LO ? MT * 32 * kLdsRow : 1];
+template <class T, int MT, int NT, int WAVES, bool LO>
+__global__ __launch_bounds__(WAVES * kWave) void bgemm_kernel(const T *At, long nkc, const uint16_t *vthi,
+                                                              const uint16_t *vtlo, long ldk, int ksplit,
+                                                              long slab_chunks, float *P, long ldn, int mpad) {
+  constexpr int NTHR = WAVES * kWave;
+  constexpr int NV = LO ? 2 : 1;
+  constexpr int VPIECES = MT * 32 * 8;  // 16-B pieces of one V^T chunk (hi or lo)
+  constexpr int VPT = (VPIECES + NTHR - 1) / NTHR;
+  __shared__ uint16_t sh[2][NV][MT * 32 * kLdsRow];
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-  const long n0 = (long)blockIdx.x * (4 * 32 * NT) + wave * (32 * NT);
-  const long kb = (long)blockIdx.y * kslab;
-  const long ke = std::min(kb + kslab, lda);
+  const int slab = (int)(blockIdx.x % ksplit);
+  const long rb = blockIdx.x / ksplit;
+  const long band0 = (rb * WAVES + wave) * NT;
+  const long c0 = slab * slab_chunks;
+  const long c1 = std::min(c0 + slab_chunks, nkc);
+  const uint16_t *vsrc[NV];
+  vsrc[0] = vthi;
+  if constexpr (LO) vsrc[NV - 1] = vtlo;
+
   f32x16_t acc[MT][NT];
 #pragma unroll
   for (int mt = 0; mt < MT; ++mt)
@@ -134,43 +160,75 @@ __global__ __launch_bounds__(kBlock) void bgemm_kernel(const T *A, long lda, con
 #pragma unroll
       for (int v = 0; v < 16; ++v) acc[mt][nt][v] = 0.f;
 
-  for (long kc = kb; kc < ke; kc += kKC) {
-    // A fragments of this chunk: 4 k-steps of 16 per n-tile
-    u32x4 af[NT][4];
+  u32x4 af[NT][4], an[NT][4];
+  u32x4 vr[NV][VPT];
+  auto load_a = [&](long c, u32x4 (&dst)[NT][4]) {
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) {
-      const T *row = A + (size_t)(n0 + nt * 32 + (lane % 32)) * lda + kc + 8 * (lane / 32);
+      const T *tile = At + ((size_t)(band0 + nt) * nkc + c) * kTile + lane * 8;
 #pragma unroll
-      for (int ks = 0; ks < 4; ++ks) af[nt][ks] = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(row + 16 * ks));
+      for (int ks = 0; ks < 4; ++ks)
+        dst[nt][ks] = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(tile + ks * 512));
     }
-    // stage V^T chunk into LDS: MT*32 rows x 64 halves (8 pieces of 16 B per row)
-    for (int p = threadIdx.x; p < MT * 32 * 8; p += kBlock) {
-      const int r = p / 8, part = p % 8;
-      *reinterpret_cast<u32x4 *>(sh_hi + r * kLdsRow + part * 8) =
-          *reinterpret_cast<const u32x4 *>(vthi + (size_t)r * ldk + kc + part * 8);
-      if constexpr (LO)
-        *reinterpret_cast<u32x4 *>(sh_lo + r * kLdsRow + part * 8) =
-            *reinterpret_cast<const u32x4 *>(vtlo + (size_t)r * ldk + kc + part * 8);
+  };
+  auto load_v = [&](long c) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q)
+#pragma unroll
+      for (int u = 0; u < VPT; ++u) {
+        const int p = threadIdx.x + u * NTHR;
+        if (VPIECES % NTHR == 0 || p < VPIECES)
+          vr[q][u] = *reinterpret_cast<const u32x4 *>(vsrc[q] + (size_t)(p >> 3) * ldk + c * kKC + (p & 7) * 8);
+      }
+  };
+  auto store_v = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q)
+#pragma unroll
+      for (int u = 0; u < VPT; ++u) {
+        const int p = threadIdx.x + u * NTHR;
+        if (VPIECES % NTHR == 0 || p < VPIECES)
+          *reinterpret_cast<u32x4 *>(&sh[buf][q][(p >> 3) * kLdsRow + (p & 7) * 8]) = vr[q][u];
+      }
+  };
+
+  if (c0 < c1) {
+    load_a(c0, af);
+    load_v(c0);
+    store_v(0);
+  }
+  // Wait for every prologue load here: the loop below then only waits for
+  // the prefetch it issues itself (without this, the wait inserted for the
+  // prologue's fragments inside the loop also drains the next chunk's loads).
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  for (long c = c0; c < c1; ++c) {
+    const int cur = (int)((c - c0) & 1);
+    const bool more = c + 1 < c1;
+    if (more) {
+      load_a(c + 1, an);
+      load_v(c + 1);
     }
-    __syncthreads();
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) {
+      const int off = (lane % 32) * kLdsRow + 16 * ks + 8 * (lane / 32);
 #pragma unroll
-      for (int mt = 0; mt < MT; ++mt) {
-        const int off = (mt * 32 + lane % 32) * kLdsRow + 16 * ks + 8 * (lane / 32);
-        const u32x4 vh = *reinterpret_cast<const u32x4 *>(sh_hi + off);
+      for (int q = 0; q < NV; ++q)
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = mfma32<T>(vh, af[nt][ks], acc[mt][nt]);
-        if constexpr (LO) {
-          const u32x4 vl = *reinterpret_cast<const u32x4 *>(sh_lo + off);
+        for (int mt = 0; mt < MT; ++mt) {
+          const u32x4 vv = *reinterpret_cast<const u32x4 *>(&sh[cur][q][mt * 32 * kLdsRow + off]);
 #pragma unroll
-          for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = mfma32<T>(vl, af[nt][ks], acc[mt][nt]);
+          for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = mfma32<T>(vv, af[nt][ks], acc[mt][nt]);
         }
-      }
     }
+    if (more) store_v(cur ^ 1);
     __syncthreads();
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks) af[nt][ks] = an[nt][ks];
   }
-  float *Pk = P + (size_t)blockIdx.y * mpad * ldn;
+  float *Pk = P + (size_t)slab * mpad * ldn;
 #pragma unroll
   for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
@@ -178,8 +236,39 @@ __global__ __launch_bounds__(kBlock) void bgemm_kernel(const T *A, long lda, con
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
         const int m = mt * 32 + 8 * (v / 4) + 4 * (lane / 32) + (v % 4);
-        Pk[(size_t)m * ldn + n0 + nt * 32 + lane % 32] = acc[mt][nt][v];
+        Pk[(size_t)m * ldn + (band0 + nt) * 32 + lane % 32] = acc[mt][nt][v];
       }
+}
+
+// Shared adjacency from a factor, written in the tiled order (one
+// workgroup per 32 x 64 tile; the same arithmetic as gen_adj_kernel).
+template <class S>
+__global__ __launch_bounds__(kBlock) void gen_adj_tiled_kernel(const float *X, const float *w, int K, long N, long nkc,
+                                                               long tile0, S *At, float inv_scale) {
+  const long tile = tile0 + blockIdx.x;
+  const long band = tile / nkc, kc = tile % nkc;
+  S *dst = At + (size_t)tile * kTile;
+  for (int o = threadIdx.x; o < kTile; o += kBlock) {
+    const int lane = (o >> 3) & 63, ks = o >> 9;
+    const long i = band * 32 + (lane & 31);
+    const long j = kc * kKC + ks * 16 + 8 * (lane >> 5) + (o & 7);
+    float acc = 0.f;
+    if (i < N && j < N) {
+      const float *Xi = X + i * K, *Xj = X + j * K;
+      for (int m = 0; m < K; ++m) acc = rfma<float>(w[m], Xi[m] * Xj[m], acc);
+    }
+    dst[o] = store_value<float, S>(acc, inv_scale);
+  }
+}
+
+// Tiled shared adjacency -> real rows (dequantized).
+template <class S>
+__global__ __launch_bounds__(kBlock) void read_tiled_rows_kernel(const S *At, long nkc, long ncols, long row0,
+                                                                 long nrows, float scale, float *dst) {
+  const long r = blockIdx.y;
+  if (r >= nrows) return;
+  for (long j = (long)blockIdx.x * kBlock + threadIdx.x; j < ncols; j += (long)gridDim.x * kBlock)
+    dst[(size_t)r * ncols + j] = load_value<float, S>(At[tiled_offset(row0 + r, j, nkc)], scale);
 }
 
 // ---------------------------------------------------------------------
@@ -662,8 +751,8 @@ struct kmt_batch {
   float t1 = 0.f;
   int begun = 0;
   // tiling (shared) / rows (grouped)
-  int tile_nt = 1, ksplit = 4, lo = 1;
-  double tune_ms[16] = {0};  // (nt in 1,2,4) x (split in 1,2,4,8,16)
+  int tile_rows = 2, ksplit = 8, lo = 1;
+  double tune_ms[16] = {0};  // (tile_rows in 1,2,4) x (split in 1,2,4,8,16)
   hipGraphExec_t g1 = nullptr, g8 = nullptr;
   // proposer
   uint16_t *mlp_w = nullptr;
@@ -734,7 +823,7 @@ struct kmt_batch {
       if (mt == 3) mt = 4;
       mpad = 32 * mt;
       lda_shared = rup(Nmax, 64);
-      rows_shared = ldn;  // multiple of 256 >= n0 + 4*32*NT of the last workgroup
+      rows_shared = ldn;  // multiple of 512: whole workgroups of every product variant
       KMT_TRY(alloc(&vthi, (size_t)mpad * lda_shared));
       KMT_TRY(alloc(&vtlo, (size_t)mpad * lda_shared));
     }
@@ -806,7 +895,18 @@ struct kmt_batch {
     const int bi = shared ? 0 : b;
     const long lda = a_ld[bi];
     const float inv = (float)(1.0 / sc);
-    for (long r0 = 0; r0 < N && e == hipSuccess; r0 += 4096) {
+    if (shared && e == hipSuccess) {
+      const long nkc = lda_shared / kKC, ntiles = (rows_shared / 32) * nkc;
+      for (long t0 = 0; t0 < ntiles && e == hipSuccess; t0 += 65536) {
+        const unsigned nb = (unsigned)std::min<long>(65536, ntiles - t0);
+        if (storage == KMT_ADJ_BF16)
+          gen_adj_tiled_kernel<bf16_t><<<nb, kBlock, 0, stream>>>(Xd, wd, K, N, nkc, t0, (bf16_t *)A, inv);
+        else
+          gen_adj_tiled_kernel<f16_t><<<nb, kBlock, 0, stream>>>(Xd, wd, K, N, nkc, t0, (f16_t *)A, inv);
+        e = hipGetLastError();
+      }
+    }
+    for (long r0 = 0; !shared && r0 < N && e == hipSuccess; r0 += 4096) {
       const long nr = std::min<long>(4096, N - r0);
       if (storage == KMT_ADJ_BF16)
         gen_adj_kernel<float, bf16_t><<<(unsigned)nr, kBlock, 0, stream>>>(Xd, wd, K, N, r0, (bf16_t *)A + a_off[bi] + (size_t)r0 * lda, lda, inv);
@@ -836,7 +936,12 @@ struct kmt_batch {
     float *stage = nullptr;
     KMT_TRY(alloc(&stage, (size_t)(i1 - i0) * N));
     dim3 grid((unsigned)std::min<long>((N + kBlock - 1) / kBlock, 64), (unsigned)(i1 - i0));
-    if (storage == KMT_ADJ_BF16)
+    if (shared) {
+      if (storage == KMT_ADJ_BF16)
+        read_tiled_rows_kernel<bf16_t><<<grid, kBlock, 0, stream>>>((const bf16_t *)A, lda_shared / kKC, N, i0, i1 - i0, mscale[0], stage);
+      else
+        read_tiled_rows_kernel<f16_t><<<grid, kBlock, 0, stream>>>((const f16_t *)A, lda_shared / kKC, N, i0, i1 - i0, mscale[0], stage);
+    } else if (storage == KMT_ADJ_BF16)
       read_rows_kernel<float, bf16_t><<<grid, kBlock, 0, stream>>>((const bf16_t *)A + a_off[bi] + (size_t)i0 * lda, lda, N, i1 - i0, mscale[bi], stage);
     else if (storage == KMT_ADJ_F16)
       read_rows_kernel<float, f16_t><<<grid, kBlock, 0, stream>>>((const f16_t *)A + a_off[bi] + (size_t)i0 * lda, lda, N, i1 - i0, mscale[bi], stage);
@@ -860,19 +965,22 @@ struct kmt_batch {
   }
 
   // ---- coupling launches ----
-  template <class T, int MT, int NT, bool LO> void gemm_v() {
-    const long tn = 4 * 32 * NT;
-    const long kslab = rup((lda_shared + ksplit - 1) / ksplit, kKC);
-    dim3 grid((unsigned)((Nmax + tn - 1) / tn), (unsigned)ksplit);
-    bgemm_kernel<T, MT, NT, LO><<<grid, kBlock, 0, stream>>>((const T *)A, lda_shared, vthi, vtlo, lda_shared, kslab, P, ldn, mpad);
+  // shared product variants: rows of A per workgroup = 128 tile_rows (1: 4
+  // waves x 1 band, 2: 8 waves x 1 band, 4: 8 waves x 2 bands)
+  template <class T, int MT, int NT, int WAVES, bool LO> void gemm_v() {
+    const long nkc = lda_shared / kKC;
+    const long slab_chunks = (nkc + ksplit - 1) / ksplit;
+    const long nrb = rows_shared / (32L * NT * WAVES);
+    bgemm_kernel<T, MT, NT, WAVES, LO><<<(unsigned)(nrb * ksplit), WAVES * kWave, 0, stream>>>(
+        (const T *)A, nkc, vthi, vtlo, lda_shared, ksplit, slab_chunks, P, ldn, mpad);
   }
   template <class T, int MT> void gemm_mt() {
-    if (tile_nt == 4) {
-      if (lo) gemm_v<T, MT, 4, true>(); else gemm_v<T, MT, 4, false>();
-    } else if (tile_nt == 2) {
-      if (lo) gemm_v<T, MT, 2, true>(); else gemm_v<T, MT, 2, false>();
+    if (tile_rows == 4) {
+      if (lo) gemm_v<T, MT, 2, 8, true>(); else gemm_v<T, MT, 2, 8, false>();
+    } else if (tile_rows == 2) {
+      if (lo) gemm_v<T, MT, 1, 8, true>(); else gemm_v<T, MT, 1, 8, false>();
     } else {
-      if (lo) gemm_v<T, MT, 1, true>(); else gemm_v<T, MT, 1, false>();
+      if (lo) gemm_v<T, MT, 1, 4, true>(); else gemm_v<T, MT, 1, 4, false>();
     }
   }
   template <class T> void gemm_t() {
@@ -1180,9 +1288,10 @@ struct kmt_batch {
     return KMT_OK;
   }
 
-  // Tile autotuner of the shared (MFMA) product: n-tiles per wave {1, 2} x
-  // split-K {1, 2, 4, 8, 16}.  The split changes the order in which partial
-  // sums are added (results agree to FP32 rounding).
+  // Tile autotuner of the shared (MFMA) product: rows per workgroup {128,
+  // 256, 512} x split-K {1, 2, 4, 8, 16}.  The split changes the order in
+  // which partial sums are added (results agree to FP32 rounding); the rows
+  // per workgroup do not change results.
   int autotune(int reps, int *best_nt, int *best_split, double *best_ms) {
     if (!shared) return KMT_OK;
     if (!A) return kmt_fail(KMT_ERR_STATE, "no adjacency");
@@ -1196,7 +1305,7 @@ struct kmt_batch {
     int idx = 0;
     for (int nt = 1; nt <= 4; nt *= 2)
       for (int si = 0; si < 5; ++si, ++idx) {
-        tile_nt = nt;
+        tile_rows = nt;
         ksplit = splits[si];
         gemm_t_any();
         HIP_TRY(hipEventRecord(e0, stream));
@@ -1210,7 +1319,7 @@ struct kmt_batch {
       }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
-    tile_nt = bnt;
+    tile_rows = bnt;
     ksplit = bks;
     drop_graphs();
     if (best_nt) *best_nt = bnt;

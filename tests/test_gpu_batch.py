@@ -77,6 +77,10 @@ def test_shared_mfma_matches_single_runs(km, orc, B, N, storage, normal):
             bt.set_instance(b, om_all[b], th_all[b], cs[b], 0.0, 0.01)
         bt.set_factor(-1, X, w)
         Mq = bt.adjacency_rows(-1, 0, N)
+        # the stored (tiled, 16-bit) matrix read back row-major: the FP32
+        # factor product rounded once to 8 (bf16) / 11 (f16) bits
+        M32 = orc.adj_rows(X, w, 0, N)
+        assert np.all(np.abs(Mq - M32) <= np.abs(M32) * 2.0 ** -8 + np.abs(M32).max() * 2.0 ** -20)
         res = bt.run(te, t1, ring_rows=64, chunk=8)
         finals = [bt.get_instance(b) for b in range(B)]
     assert res["status"].active == 0 and res["status"].errors == 0
