@@ -44,6 +44,10 @@ CONFIGS = {
     # -A, the low-rank factor only (N x K), coupled through the factor
     "fp32-n200000-lowrank": dict(N=200000, K=5, seed=3, prec="f32", normal=False, init=0.0, c1=3.0, atol=1e-6,
                                  t1=20.0, dt=0.5, storage="factored"),
+    # configs[2] in the north_star's literal form: adjacency tiles rebuilt in
+    # registers from X at every stage (dense semantics, VALU-bound)
+    "fp32-n200000-rebuild": dict(N=200000, K=5, seed=3, prec="f32", normal=False, init=0.0, c1=3.0, atol=1e-6,
+                                 t1=20.0, dt=0.5, storage="rebuild"),
     # configs[2]'s check run: the same A stored explicitly in FP16 (80 GB)
     "fp32-n200000-f16": dict(N=200000, K=5, seed=3, prec="f32", normal=False, init=0.0, c1=3.0, atol=1e-6,
                              t1=20.0, dt=0.5, storage="f16"),
@@ -65,11 +69,11 @@ BATCH_CONFIGS = {
 HBM_PEAK = 8.0e12
 
 
-def profiled_traffic(config):
-    """HBM bytes per stage launch measured by the rocprofv3 FETCH_SIZE /
-    WRITE_SIZE runs recorded in profiles/stage_kernel_<config>.txt (None if
-    that workload has not been profiled)."""
-    path = os.path.join(ROOT, "profiles", "stage_kernel_" + config.replace("-", "_") + ".txt")
+def profiled_traffic(config, kernel="stage_kernel"):
+    """HBM bytes per launch of the dominant kernel measured by the rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs recorded in profiles/<kernel>_<config>.txt
+    (None if that workload has not been profiled)."""
+    path = os.path.join(ROOT, "profiles", kernel + "_" + config.replace("-", "_") + ".txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -94,7 +98,9 @@ def gpu_run(cfg, steps, warmup):
     it.set_factor(X, w)
     t0 = time.time()
     if cfg["storage"] == "factored":
-        it.use_factored(True)
+        it.use_factored("contract")
+    elif cfg["storage"] == "rebuild":
+        it.use_factored("rebuild")
     else:
         it.build_adjacency(cfg["storage"])
     gen_s = time.time() - t0
@@ -127,7 +133,7 @@ def cpu_baseline(cfg, max_steps):
     w = orc.factor_weights(N, K, True, dtype=dtype)
     # N^2 does not fit the host for the N=200000 cases: the oracle runs the
     # reference's own -A path (coupling through the factor) there
-    lowrank = cfg["storage"] in ("factored", "f16")
+    lowrank = cfg["storage"] in ("factored", "f16", "rebuild")
     adj = orc.Adj(2, X=X, w=w) if lowrank else orc.Adj(0, M=orc.adj_rows(X, w, 0, N))
     dt = dtype(cfg["dt"])
     te = (dt * np.arange(1, int(dtype(cfg["t1"]) / dt) + 1)).astype(dtype)
@@ -149,7 +155,7 @@ def uneven_sizes(B, seed=147):
     return [int(5000 * rng.integers(1, 9)) for _ in range(B)]
 
 
-def batch_run(name, steps, warmup, proposer):
+def make_batch(name, proposer):
     import kuramoto_amd as km
 
     cfg = BATCH_CONFIGS[name]
@@ -169,6 +175,27 @@ def batch_run(name, steps, warmup, proposer):
     if proposer:
         w = np.load(os.path.join(ROOT, "kuramoto_amd", "proposer_weights.npz"))["w16"]
         bt.set_proposer(w, True)
+    return bt, Ns, tune
+
+
+def batch_to_time(name, proposer, t_end):
+    """Whole integration of every instance from t=0 to t_end (host wall
+    clock around the advance loop, adjacency already resident)."""
+    bt, Ns, _ = make_batch(name, proposer)
+    dt = np.float32(BATCH_CONFIGS[name]["dt"])
+    te = (dt * np.arange(1, int(np.float32(t_end) / dt) + 1)).astype(np.float64)
+    t0 = time.perf_counter()
+    res = bt.run(te, t_end, ring_rows=1024, chunk=64)
+    secs = time.perf_counter() - t0
+    bt.close()
+    st = res["status"]
+    return dict(seconds=secs, r=res["r"], rhs_evals=st.rhs_evals, accepted=st.accepted, rejected=st.rejected,
+                osc_steps=st.osc_steps, proposer_accepts=st.proposer_accepts, proposer_rejects=st.proposer_rejects)
+
+
+def batch_run(name, steps, warmup, proposer):
+    cfg = BATCH_CONFIGS[name]
+    bt, Ns, tune = make_batch(name, proposer)
     dt = np.float32(cfg["dt"])
     te = (dt * np.arange(1, int(np.float32(cfg["t1"]) / dt) + 1)).astype(np.float64)
     bt.begin(te, cfg["t1"], ring_rows=1024)
@@ -198,6 +225,8 @@ def plain_attempt_ms(N, steps=10):
 
 
 def batch_main(args):
+    import kuramoto_amd as km
+
     cfg = BATCH_CONFIGS[args.config]
     r = batch_run(args.config, args.steps, args.warmup, proposer=not cfg["shared"])
     st, st0 = r["st"], r["st0"]
@@ -217,19 +246,22 @@ def batch_main(args):
                  "tile": {"n_tiles_per_wave": r["tune"][0], "split_k": r["tune"][1]}}
     else:
         alg = sum(2 * n * n + 3 * 4 * n for n in Ns)
-        base = batch_run(args.config, args.steps, args.warmup, proposer=False)
-        b_steps = base["st"].osc_steps - base["st0"].osc_steps
-        extra = {"standard_controller": {"osc_steps_per_s": b_steps / (base["region_ms"] / 1e3),
-                                         "rhs_evals": base["st"].rhs_evals - base["st0"].rhs_evals,
-                                         "rejected": base["st"].rejected - base["st0"].rejected,
-                                         "accepted": base["st"].accepted - base["st0"].accepted},
-                 "learned_proposer": {"rhs_evals": st.rhs_evals - st0.rhs_evals,
-                                      "rejected": st.rejected - st0.rejected,
-                                      "accepted": st.accepted - st0.accepted,
-                                      "proposer_accepts": st.proposer_accepts - st0.proposer_accepts,
-                                      "proposer_rejects": st.proposer_rejects - st0.proposer_rejects},
-                 "sum_N": int(sum(Ns))}
+        # the learned proposer changes how many attempts reach a given t, so
+        # the two controllers are compared on whole runs to the same end time
+        t_end = args.batch_t_end
+        runs = {m: batch_to_time(args.config, m == "learned_proposer", t_end)
+                for m in ("standard_controller", "learned_proposer")}
+        dr = max(float(np.max(np.abs(a - b))) for a, b in zip(runs["standard_controller"]["r"],
+                                                               runs["learned_proposer"]["r"]))
+        for m in runs:
+            runs[m].pop("r")
+            runs[m]["osc_steps_per_s"] = runs[m]["osc_steps"] / runs[m]["seconds"]
+        extra = {f"whole_run_to_t{t_end:g}": runs, "max_abs_dr_learned_vs_standard": dr,
+                 "speedup_learned_vs_standard": runs["standard_controller"]["seconds"]
+                 / runs["learned_proposer"]["seconds"], "sum_N": int(sum(Ns))}
     achieved = alg / (r["coupling_ms"] / 1e3)
+    copy_gbps, read_gbps = km.bandwidth_probe(8 << 30, 5)
+    kname = "bgemm_kernel" if cfg["shared"] else "bgemv_kernel"
     out = {
         "metric": "osc*steps/s (sum over instances of N_b x accepted steps per second)",
         "value": osc_steps / secs,
@@ -241,9 +273,10 @@ def batch_main(args):
         "config": {"workload": args.config, "B": cfg["B"], "instance_steps_in_region": st.accepted - st0.accepted,
                    **extra},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None,
-                     "kernel": "bgemm_kernel (MFMA)" if cfg["shared"] else "bgemv_kernel (grouped)",
-                     "bytes_per_launch": alg, "avg_launch_ms": r["coupling_ms"]},
+                     "frac": achieved / HBM_PEAK, "traffic": profiled_traffic(args.config, kname),
+                     "kernel": kname + (" (MFMA, hi+lo vectors)" if cfg["shared"] else " (grouped gemv-pairs)"),
+                     "bytes_per_launch": alg, "avg_launch_ms": r["coupling_ms"],
+                     "measured_read_stream_gbps": read_gbps, "measured_copy_gbps": copy_gbps},
     }
     print(json.dumps(out), flush=True)
 
@@ -255,6 +288,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="fp64-n50000", choices=sorted(CONFIGS) + sorted(BATCH_CONFIGS))
     ap.add_argument("--cpu-steps", type=int, default=20, help="accepted steps of the CPU baseline sample (0: skip)")
+    ap.add_argument("--batch-t-end", type=float, default=10.0,
+                    help="uneven batch: end time of the whole runs that compare the two step controllers")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (see DESIGN.md, "
@@ -275,6 +310,17 @@ def main():
     else:
         alg_bytes = r["elem_bytes"] * N * N + N * (2 * s_y + s_y)
     achieved = alg_bytes / (r["stage_ms"] / 1e3)
+    roof = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK}
+    if cfg["storage"] == "rebuild":
+        # (K products + K fmas to form M_ij, 2 fmas into C and S) per pair:
+        # 3K + 4 flops in 2K + 2 VALU instructions per (i, j), against the
+        # published FP32 vector peak (157.3 TFLOP/s, all-FMA instruction mix)
+        flops = (3 * cfg["K"] + 4) * N * N
+        tf = flops / (r["stage_ms"] / 1e3) / 1e12
+        roof = {"bound": "valu", "achieved": tf, "peak": 157.3, "unit": "TFLOP/s", "frac": tf / 157.3,
+                "flops_per_launch": flops,
+                "valu_instructions_per_launch": (2 * cfg["K"] + 2) * N * N}
     out = {
         "metric": "osc*steps/s (N x accepted DP45 steps per second)",
         "value": N * accepted / secs,
@@ -289,15 +335,18 @@ def main():
         "dtype": "f64" if s_y == 8 else "f32",
         "data": "synthetic (Philox Gaussian factor, glibc-rand frequencies/phases, as the CLI generates)",
         "config": {"workload": f"{args.config}: N={N}, K={cfg['K']}, "
-                               + (f"low-rank factor only (-A)" if cfg["storage"] == "factored" else
+                               + ("low-rank factor only (-A)" if cfg["storage"] == "factored" else
+                                  "-A, tiles rebuilt in registers" if cfg["storage"] == "rebuild" else
                                   f"dense adjacency {cfg['storage']} ({r['elem_bytes']} B/elem)")
                                + f", atol={cfg['atol']}, c={cfg['c1']}",
                    "N": N, "accepted_in_region": accepted,
                    "rejected_in_region": r["st"].rejected - r["st0"].rejected,
                    "t_after": r["st"].t, "adjacency_build_s": round(r["gen_s"], 3)},
-        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": profiled_traffic(args.config),
-                     "kernel": ("lowrank partial+reduce+expand (one RHS)" if cfg["storage"] == "factored"
+        "roofline": {**roof,
+                     "traffic": profiled_traffic(args.config, "lowrank_stage_kernel" if cfg["storage"] == "factored"
+                                                 else "stage_kernel"),
+                     "kernel": ("lowrank_stage_kernel (+ gram at stage 2; one RHS)" if cfg["storage"] == "factored"
+                                else "rebuild_stage_kernel (tiles rebuilt in registers)" if cfg["storage"] == "rebuild"
                                 else "stage_kernel (fused gemv-pair + DP45 stage epilogue)"),
                      "bytes_per_launch": alg_bytes, "avg_launch_ms": r["stage_ms"],
                      "variant": {"rows_per_workgroup": r["rows"], "nontemporal_loads": r["nt"]},

@@ -114,9 +114,15 @@ int kmt_set_factor(kmt_ctx *ctx, const void *X, const void *w, int K);
  * kuramoto.cu:145-161): M_ij = sum_m fma(w_m, X_im X_jm, .) in the working
  * precision, stored as `storage`.  Exactly symmetric. */
 int kmt_build_adjacency(kmt_ctx *ctx, int storage);
-/* -A: couple through the factor (O(N K) per RHS) instead of a dense matrix
- * (makecoupling's m1/m2 branch, kuramoto.cu:62-108). */
-int kmt_use_factored(kmt_ctx *ctx, int on);
+/* -A: couple through the factor instead of a dense matrix (makecoupling's
+ * m1/m2 branch, kuramoto.cu:62-108).  mode KMT_LOWRANK_CONTRACT: X diag(w)
+ * (X^T v), O(N K) per RHS (the reference's algorithm); KMT_LOWRANK_REBUILD:
+ * adjacency tiles M_ij rebuilt in registers from X for every stage, O(N^2 K)
+ * VALU work, no N x N storage, results equal to the dense path on the
+ * generated matrix (K <= 16); 0 returns to the dense matrix. */
+#define KMT_LOWRANK_CONTRACT 1
+#define KMT_LOWRANK_REBUILD 2
+int kmt_use_factored(kmt_ctx *ctx, int mode);
 /* Read back rows [i0, i1) of the dense adjacency (dequantized, context
  * precision, N values per row) — the adj.dat writer (kuramoto.cu:570-578). */
 int kmt_get_adjacency_rows(kmt_ctx *ctx, long i0, long i1, void *out);

@@ -250,7 +250,12 @@ class Integrator:
         _check(self.lib.kmt_build_adjacency(self.ctx, STORAGE[storage]), "kmt_build_adjacency")
 
     def use_factored(self, on=True):
-        _check(self.lib.kmt_use_factored(self.ctx, int(on)), "kmt_use_factored")
+        """-A coupling: True / "contract" = X diag(w) (X^T v); "rebuild" =
+        adjacency tiles rebuilt in registers every stage; False = dense."""
+        mode = {"contract": 1, "rebuild": 2}.get(on, None) if isinstance(on, str) else int(on)
+        if mode is None:
+            raise ValueError(f"unknown low-rank mode {on!r}")
+        _check(self.lib.kmt_use_factored(self.ctx, mode), "kmt_use_factored")
 
     def adjacency_rows(self, i0, i1):
         out = np.zeros((i1 - i0, self.N), self.dtype)

@@ -132,10 +132,11 @@ template <class R> struct Engine : EngineBase {
   void *A = nullptr;
   size_t A_bytes = 0;
   R *X = nullptr, *w = nullptr;
+  R *XT = nullptr;  // transposed factor [K][ldv] for the register-rebuild form
   int K = 0;
-  int factored = 0;
+  int factored = 0;  // 0 dense A, KMT_LOWRANK_CONTRACT, KMT_LOWRANK_REBUILD
   double *lr_part = nullptr, *lr_G = nullptr;
-  int lr_nblk = 0;
+  int lr_nblk = 0, lr_grid = 0;
   std::vector<R> X_host, w_host;
 
   // run
@@ -156,7 +157,7 @@ template <class R> struct Engine : EngineBase {
   void release() {
     if (stream) hipStreamSynchronize(stream);
     drop_graphs();
-    R *vecs[] = {y, ylast, ynew, omega, kbuf, cs[0], cs[1], cs_scr, vec_scr, f_scr, pairs, k_scr, X, w, t_eval, ring, r_ring, times};
+    R *vecs[] = {y, ylast, ynew, omega, kbuf, cs[0], cs[1], cs_scr, vec_scr, f_scr, pairs, k_scr, X, w, XT, t_eval, ring, r_ring, times};
     for (R *p : vecs)
       if (p) hipFree(p);
     double *dv[] = {err_part, ord_scr, lr_part, lr_G, ord_part};
@@ -368,6 +369,7 @@ template <class R> struct Engine : EngineBase {
     drop_graphs();
     if (X) { hipFree(X); X = nullptr; }
     if (w) { hipFree(w); w = nullptr; }
+    if (XT) { hipFree(XT); XT = nullptr; }
     if (lr_part) { hipFree(lr_part); lr_part = nullptr; }
     if (lr_G) { hipFree(lr_G); lr_G = nullptr; }
     K = k_;
@@ -381,10 +383,17 @@ template <class R> struct Engine : EngineBase {
       HIP_TRY(hipMemcpyAsync(w, wv, sizeof(R) * K, hipMemcpyHostToDevice, stream));
     }
     lr_nblk = (int)std::min<long>((N + kBlock - 1) / kBlock, 512);
-    HIP_TRY(hipMalloc((void **)&lr_part, sizeof(double) * 2 * kk * lr_nblk));
+    lr_grid = (int)std::min<long>((N + kBlock - 1) / kBlock, kLrMaxBlocks);
+    const int nparts = std::max(lr_nblk, 2 * lr_grid);
+    HIP_TRY(hipMalloc((void **)&lr_part, sizeof(double) * 2 * kk * nparts));
+    HIP_TRY(hipMemsetAsync(lr_part, 0, sizeof(double) * 2 * kk * nparts, stream));
     HIP_TRY(hipMalloc((void **)&lr_G, sizeof(double) * 2 * kk));
     HIP_TRY(hipMemsetAsync(lr_G, 0, sizeof(double) * 2 * kk, stream));
     HIP_TRY(hipStreamSynchronize(stream));
+    if (factored == KMT_LOWRANK_REBUILD) {  // keep the mode: transpose the new factor
+      factored = KMT_LOWRANK_CONTRACT;
+      return use_factored(KMT_LOWRANK_REBUILD);
+    }
     return KMT_OK;
   }
 
@@ -424,7 +433,19 @@ template <class R> struct Engine : EngineBase {
   }
 
   int use_factored(int on) override {
+    if (on < 0 || on > KMT_LOWRANK_REBUILD) return fail(KMT_ERR_ARG, "unknown low-rank mode %d", on);
     if (on && !X) return fail(KMT_ERR_STATE, "-A needs kmt_set_factor first");
+    if (on == KMT_LOWRANK_REBUILD) {
+      if (K > kLrMaxRank) return fail(KMT_ERR_ARG, "register-rebuilt tiles support K <= %d (K=%d)", kLrMaxRank, K);
+      if (!XT) {
+        std::vector<R> t((size_t)std::max(K, 1) * ldv, (R)0);
+        for (long i = 0; i < N; ++i)
+          for (int m = 0; m < K; ++m) t[(size_t)m * ldv + i] = X_host[(size_t)i * K + m];
+        HIP_TRY(hipMalloc((void **)&XT, sizeof(R) * t.size()));
+        HIP_TRY(hipMemcpyAsync(XT, t.data(), sizeof(R) * t.size(), hipMemcpyHostToDevice, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+      }
+    }
     if (factored != on) drop_graphs();
     factored = on;
     return KMT_OK;
@@ -511,7 +532,59 @@ template <class R> struct Engine : EngineBase {
     }
   }
 
+  bool lr_fused() const { return K <= kLrMaxRank; }
+  // partial-sum buffers of the fused low-rank path (used alternately)
+  double *lr_p(int which) const { return lr_part + (size_t)which * 2 * std::max(K, 1) * lr_grid; }
+
+  template <int KT> void launch_gram_t(const R *cs_in, int which, int check) {
+    lowrank_gram_kernel<R, KT><<<(unsigned)lr_grid, kBlock, 0, stream>>>(X, K, N, cs_in, ldv, lr_p(which), st, check);
+  }
+  void launch_gram(const R *cs_in, int which, int check) {
+    if (K <= 8) launch_gram_t<8>(cs_in, which, check);
+    else launch_gram_t<16>(cs_in, which, check);
+  }
+  // stage KIND from the partials in lr_p(pin); with NEXT, the partials of
+  // its output vector into lr_p(pin ^ 1)
+  template <int KIND, int KT, bool NEXT> void launch_lr_stage_t(const Vecs<R> &v, int pin) {
+    lowrank_stage_kernel<R, KIND, KT, NEXT><<<(unsigned)lr_grid, kBlock, 0, stream>>>(X, K, lr_p(pin), lr_grid, w, v,
+                                                                                       lr_p(pin ^ 1));
+  }
+  template <int KIND, bool NEXT> void launch_lr_stage(const Vecs<R> &v, int pin) {
+    if (K <= 8) launch_lr_stage_t<KIND, 8, NEXT>(v, pin);
+    else launch_lr_stage_t<KIND, 16, NEXT>(v, pin);
+  }
+  // one attempt's six stages on the fused low-rank path: partials of the
+  // stage-2 vector, then one launch per stage that also writes the partials
+  // of the next stage's vector
+  void launch_lr_attempt(std::vector<std::pair<hipEvent_t, hipEvent_t>> *ev) {
+    for (int s = 2; s <= 7; ++s) {
+      const R *in = cs[(s - 2) & 1];
+      R *out = cs[(s - 1) & 1];
+      Vecs<R> v = vecs(in, out, 1);
+      const int pin = (s - 2) & 1;
+      if (ev) hipEventRecord((*ev)[0].first, stream);
+      if (s == 2) launch_gram(in, 0, 1);
+      switch (s) {
+        case 2: launch_lr_stage<2, true>(v, pin); break;
+        case 3: launch_lr_stage<3, true>(v, pin); break;
+        case 4: launch_lr_stage<4, true>(v, pin); break;
+        case 5: launch_lr_stage<5, true>(v, pin); break;
+        case 6: launch_lr_stage<6, true>(v, pin); break;
+        default: launch_lr_stage<7, false>(v, pin); break;
+      }
+      if (ev) {
+        hipEventRecord((*ev)[0].second, stream);
+        ev->erase(ev->begin());
+      }
+    }
+  }
+
   template <int KIND> void launch_factored(const Vecs<R> &v) {
+    if (lr_fused()) {
+      launch_gram(v.cs_in, 0, v.check_state && KIND >= 2);
+      launch_lr_stage<KIND, false>(v, 0);
+      return;
+    }
     const int tiles = std::max(1, (K + kRankTile - 1) / kRankTile);
     if (K > 0) {
       lowrank_partial_kernel<R><<<dim3(lr_nblk, tiles), kBlock, 0, stream>>>(X, K, N, v.cs_in, ldv, lr_part, st, v.check_state && KIND >= 2);
@@ -521,8 +594,23 @@ template <class R> struct Engine : EngineBase {
     lowrank_expand_kernel<R, KIND><<<grid, kBlock, 0, stream>>>(X, K, lr_G, v);
   }
 
+  int rebuild_rows() const { return K == 5 ? (sizeof(R) == 4 ? 16 : 8) : K <= 8 ? 8 : 4; }
+  template <int KIND> void launch_rebuild(const Vecs<R> &v) {
+    RebuildArgs<R> a;
+    a.XT = XT;
+    a.w = w;
+    a.K = K;
+    a.v = v;
+    const int rows = rebuild_rows();
+    const unsigned grid = (unsigned)((N + rows - 1) / rows);
+    if (K == 5) rebuild_stage_kernel<R, sizeof(R) == 4 ? 16 : 8, KIND, 5, true><<<grid, kBlock, 0, stream>>>(a);
+    else if (K <= 8) rebuild_stage_kernel<R, 8, KIND, 8, false><<<grid, kBlock, 0, stream>>>(a);
+    else rebuild_stage_kernel<R, 4, KIND, 16, false><<<grid, kBlock, 0, stream>>>(a);
+  }
+
   template <int KIND> void launch_kind(const Vecs<R> &v) {
-    if (factored) launch_factored<KIND>(v);
+    if (factored == KMT_LOWRANK_REBUILD) launch_rebuild<KIND>(v);
+    else if (factored) launch_factored<KIND>(v);
     else launch_dense<KIND>(v);
   }
 
@@ -540,7 +628,8 @@ template <class R> struct Engine : EngineBase {
   }
 
   int n_parts_stage7() const {
-    if (factored) return (int)((N + kBlock - 1) / kBlock);
+    if (factored == KMT_LOWRANK_REBUILD) return (int)((N + rebuild_rows() - 1) / rebuild_rows());
+    if (factored) return lr_fused() ? lr_grid : (int)((N + kBlock - 1) / kBlock);
     return (int)((N + tune_rows - 1) / tune_rows);
   }
 
@@ -604,7 +693,8 @@ template <class R> struct Engine : EngineBase {
   // One attempt: stages 2..7 (cs ping-pong 0->1->0->1->0->1), control,
   // emit/commit (writes stage-2 vector into cs[0]), finalize.
   void launch_attempt(std::vector<std::pair<hipEvent_t, hipEvent_t>> *ev = nullptr) {
-    for (int s = 2; s <= 7; ++s) {
+    if (factored == KMT_LOWRANK_CONTRACT && lr_fused()) launch_lr_attempt(ev);
+    else for (int s = 2; s <= 7; ++s) {
       const R *in = cs[(s - 2) & 1];
       R *out = cs[(s - 1) & 1];
       Vecs<R> v = vecs(in, out, 1);
@@ -1055,7 +1145,7 @@ int kmt_build_adjacency(kmt_ctx *ctx, int storage) {
 }
 int kmt_use_factored(kmt_ctx *ctx, int on) {
   CTX_CHECK;
-  return ctx->eng->use_factored(on ? 1 : 0);
+  return ctx->eng->use_factored(on);
 }
 int kmt_get_adjacency_rows(kmt_ctx *ctx, long i0, long i1, void *out) {
   CTX_CHECK;

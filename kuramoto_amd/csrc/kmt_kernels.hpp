@@ -254,6 +254,49 @@ template <class R, class S> struct DenseArgs {
   Vecs<R> v;
 };
 
+// Row sums of a stage kernel: wave shuffles, then the 4 wave sums in wave
+// order (fixed tree), scale, row epilogue; stage 7 writes one error partial
+// per workgroup.
+template <class R, class Acc, int ROWS, int KIND>
+__device__ __forceinline__ void finish_rows(Acc (&accC)[ROWS], Acc (&accS)[ROWS], long row0, R mscale,
+                                            const Vecs<R> &v) {
+  __shared__ Acc red[kBlock / kWave][2 * ROWS];
+  __shared__ double errsq[ROWS];
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1), wave = tid / kWave;
+#pragma unroll
+  for (int r = 0; r < ROWS; ++r) {
+    accC[r] = wave_sum(accC[r]);
+    accS[r] = wave_sum(accS[r]);
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) { red[wave][r] = accC[r]; red[wave][ROWS + r] = accS[r]; }
+  }
+  __syncthreads();
+  if (tid < ROWS) {
+    const long i = row0 + tid;
+    double sq = 0.0;
+    if (i < v.N) {
+      Acc C = red[0][tid], Sv = red[0][ROWS + tid];
+#pragma unroll
+      for (int w = 1; w < kBlock / kWave; ++w) { C += red[w][tid]; Sv += red[w][ROWS + tid]; }
+      const R h = (KIND >= 2) ? v.st->h : (R)0;
+      sq = row_epilogue<R, KIND>(v, i, (R)C * mscale, (R)Sv * mscale, h);
+    }
+    if constexpr (KIND == 7) errsq[tid] = sq;
+  }
+  if constexpr (KIND == 7) {
+    __syncthreads();
+    if (tid == 0) {
+      double s = 0.0;
+#pragma unroll
+      for (int r = 0; r < ROWS; ++r) s += errsq[r];
+      v.err_part[blockIdx.x] = s;
+    }
+  }
+}
+
 template <class R, class S, int ROWS, int KIND, bool NT>
 __global__ __launch_bounds__(kBlock) void stage_kernel(const DenseArgs<R, S> a) {
   if constexpr (KIND >= 2) {
@@ -325,40 +368,74 @@ __global__ __launch_bounds__(kBlock) void stage_kernel(const DenseArgs<R, S> a) 
     }
   }
 
-  __shared__ Acc red[kBlock / kWave][2 * ROWS];
-  __shared__ double errsq[ROWS];
-  const int lane = tid & (kWave - 1), wave = tid / kWave;
-#pragma unroll
-  for (int r = 0; r < ROWS; ++r) {
-    accC[r] = wave_sum(accC[r]);
-    accS[r] = wave_sum(accS[r]);
+  finish_rows<R, Acc, ROWS, KIND>(accC, accS, row0, a.mscale, a.v);
+}
+
+// ---------------------------------------------------------------------
+// -A with register-rebuilt adjacency tiles: M_ij = sum_m w_m (X_im X_jm)
+// is formed in registers for every (row, column) pair of a stage from the
+// transposed factor XT (K x ldv, zero columns past N) — no N x N matrix in
+// HBM, (2K + 2) N^2 VALU operations per RHS.  Each entry uses the dense
+// generator's formula and the columns are visited in the dense stage
+// kernel's order (VEC per 16-byte chunk, chunks strided by kBlock VEC), so
+// the result equals the dense FP32/FP64 path on the generated matrix bit
+// for bit.
+// ---------------------------------------------------------------------
+template <class R> struct RebuildArgs {
+  const R *XT;  // [K][ldv]
+  const R *w;   // [K]
+  int K;
+  Vecs<R> v;
+};
+
+// KT: rank bound of the register arrays; EXACT: K == KT (no per-term
+// guards in the unrolled inner loop).
+template <class R, int ROWS, int KIND, int KT, bool EXACT>
+__global__ __launch_bounds__(kBlock) void rebuild_stage_kernel(const RebuildArgs<R> a) {
+  if constexpr (KIND >= 2) {
+    if (a.v.check_state && (!a.v.st->active || a.v.st->commit_pending)) return;
   }
-  if (lane == 0) {
+  constexpr int VEC = 16 / sizeof(R);
+  constexpr long STRIDE = (long)kBlock * VEC;
+  const long ldv = a.v.ldv;
+  const long row0 = (long)blockIdx.x * ROWS;
+  const int K = EXACT ? KT : a.K;
+  // The row factors are the same in every lane; a lane shuffle gives them a
+  // per-lane register copy (otherwise they sit in scalar registers that
+  // spill, and every use in the inner loop reloads them).
+  const int lane = threadIdx.x & (kWave - 1);
+  R xi[ROWS][KT], wm[KT];
 #pragma unroll
-    for (int r = 0; r < ROWS; ++r) { red[wave][r] = accC[r]; red[wave][ROWS + r] = accS[r]; }
-  }
-  __syncthreads();
-  if (tid < ROWS) {
-    const long i = row0 + tid;
-    double sq = 0.0;
-    if (i < a.v.N) {
-      Acc C = red[0][tid], Sv = red[0][ROWS + tid];
+  for (int m = 0; m < KT; ++m) {
+    wm[m] = __shfl(m < K ? a.w[m] : (R)0, lane, kWave);
 #pragma unroll
-      for (int w = 1; w < kBlock / kWave; ++w) { C += red[w][tid]; Sv += red[w][ROWS + tid]; }
-      const R h = (KIND >= 2) ? a.v.st->h : (R)0;
-      sq = row_epilogue<R, KIND>(a.v, i, (R)C * a.mscale, (R)Sv * a.mscale, h);
-    }
-    if constexpr (KIND == 7) errsq[tid] = sq;
+    for (int r = 0; r < ROWS; ++r) xi[r][m] = __shfl(m < K ? a.XT[m * ldv + row0 + r] : (R)0, lane, kWave);
   }
-  if constexpr (KIND == 7) {
-    __syncthreads();
-    if (tid == 0) {
-      double s = 0.0;
+  const R *cosv = a.v.cs_in;
+  const R *sinv = a.v.cs_in + ldv;
+  R accC[ROWS], accS[ROWS];
 #pragma unroll
-      for (int r = 0; r < ROWS; ++r) s += errsq[r];
-      a.v.err_part[blockIdx.x] = s;
-    }
+  for (int r = 0; r < ROWS; ++r) { accC[r] = 0; accS[r] = 0; }
+  for (long c = (long)threadIdx.x * VEC; c < ldv; c += STRIDE) {
+    R cv[VEC], sv[VEC], xj[KT][VEC];
+    load_reals<R, VEC>(cosv + c, cv);
+    load_reals<R, VEC>(sinv + c, sv);
+#pragma unroll
+    for (int m = 0; m < KT; ++m)
+      if (m < K) load_reals<R, VEC>(a.XT + m * ldv + c, xj[m]);
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r)
+#pragma unroll
+      for (int q = 0; q < VEC; ++q) {
+        R mij = 0;
+#pragma unroll
+        for (int m = 0; m < KT; ++m)
+          if (m < K) mij = rfma<R>(wm[m], xi[r][m] * xj[m][q], mij);
+        accC[r] = rfma<R>(mij, cv[q], accC[r]);
+        accS[r] = rfma<R>(mij, sv[q], accS[r]);
+      }
   }
+  finish_rows<R, R, ROWS, KIND>(accC, accS, row0, (R)1, a.v);
 }
 
 // ---------------------------------------------------------------------
@@ -434,6 +511,145 @@ __global__ __launch_bounds__(kBlock) void lowrank_expand_kernel(const R *X, int 
     const double tot = block_sum(sq, errsq);
     if (threadIdx.x == 0) v.err_part[blockIdx.x] = tot;
   }
+}
+
+// ---------------------------------------------------------------------
+// Low-rank coupling, one launch per RHS (K <= kLrMaxRank).
+//   G = w .* X^T (cos, sin) is formed in two halves: the launch that writes
+//   a stage vector also writes one partial sum of X^T (cos, sin) per
+//   workgroup (wave shuffles, then the wave sums in order); the next launch
+//   starts with every workgroup adding those partials in workgroup order
+//   (the same order in each, so all of them hold the same G), then computes
+//   C_i = sum_m X_im G_c[m], S_i likewise and the row epilogue of its stage.
+//   lowrank_gram_kernel writes the partials of a given vector (the first
+//   stage of an attempt, a stand-alone RHS).  No cross-workgroup
+//   synchronisation inside a launch; partial buffers alternate between
+//   consecutive launches.
+// Rows are strided over a grid of at most kLrMaxBlocks workgroups.
+// ---------------------------------------------------------------------
+constexpr int kLrMaxRank = 16;
+constexpr int kLrMaxBlocks = 512;
+
+template <int KT>
+__device__ __forceinline__ void lr_write_partials(double (&acc)[2 * KT], int K, double *part) {
+  __shared__ double red[kBlock / kWave][2 * KT];
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int nv = 2 * K;
+#pragma unroll
+  for (int q = 0; q < 2 * KT; ++q)
+    if (q < nv) {
+      const double t = wave_sum(acc[q]);
+      if (lane == 0) red[wave][q] = t;
+    }
+  __syncthreads();
+  if (threadIdx.x < nv) {
+    double t = red[0][threadIdx.x];
+#pragma unroll
+    for (int q = 1; q < kBlock / kWave; ++q) t += red[q][threadIdx.x];
+    part[(size_t)blockIdx.x * nv + threadIdx.x] = t;
+  }
+}
+
+// G (2K values, interleaved c/s) from nparts <= 2 kBlock partial vectors,
+// into g (LDS).  Thread t adds partials t and t + kBlock (all loads issued
+// at once), then the sums go through the wave shuffles and the wave order.
+static_assert(kLrMaxBlocks <= 2 * kBlock, "lr_reduce covers two partials per thread");
+template <class R, int KT>
+__device__ __forceinline__ void lr_reduce(const double *part, int nparts, int K, const R *w, double *g) {
+  __shared__ double red[kBlock / kWave][2 * KT];
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int nv = 2 * K;
+  const int b0 = threadIdx.x, b1 = threadIdx.x + kBlock;
+  double t0[2 * KT], t1[2 * KT];
+#pragma unroll
+  for (int q = 0; q < 2 * KT; ++q) {
+    t0[q] = (q < nv && b0 < nparts) ? part[(size_t)b0 * nv + q] : 0.0;
+    t1[q] = (q < nv && b1 < nparts) ? part[(size_t)b1 * nv + q] : 0.0;
+  }
+#pragma unroll
+  for (int q = 0; q < 2 * KT; ++q)
+    if (q < nv) {
+      const double t = wave_sum(t0[q] + t1[q]);
+      if (lane == 0) red[wave][q] = t;
+    }
+  __syncthreads();
+  if (threadIdx.x < 2 * KT) {
+    const int q = threadIdx.x;
+    double t = 0.0;
+    if (q < nv) {
+      t = red[0][q];
+#pragma unroll
+      for (int r = 1; r < kBlock / kWave; ++r) t += red[r][q];
+      t *= (double)w[q >> 1];
+    }
+    g[q] = t;
+  }
+  __syncthreads();
+}
+
+template <class R, int KT>
+__global__ __launch_bounds__(kBlock) void lowrank_gram_kernel(const R *X, int K, long N, const R *cs, long ldv,
+                                                              double *part, const State<R> *st, int check_state) {
+  if (check_state && (!st->active || st->commit_pending)) return;
+  double acc[2 * KT];
+#pragma unroll
+  for (int q = 0; q < 2 * KT; ++q) acc[q] = 0.0;
+  for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < N; i += (long)gridDim.x * kBlock) {
+    const double c = cs[i], s = cs[ldv + i];
+#pragma unroll
+    for (int m = 0; m < KT; ++m)
+      if (m < K) {
+        const double x = X[i * K + m];
+        acc[2 * m] = __builtin_fma(x, c, acc[2 * m]);
+        acc[2 * m + 1] = __builtin_fma(x, s, acc[2 * m + 1]);
+      }
+  }
+  lr_write_partials<KT>(acc, K, part);
+}
+
+template <class R, int KIND, int KT, bool NEXT>
+__global__ __launch_bounds__(kBlock) void lowrank_stage_kernel(const R *X, int K, const double *part_in, int nparts,
+                                                               const R *w, Vecs<R> v, double *part_out) {
+  if constexpr (KIND >= 2) {
+    if (v.check_state && (!v.st->active || v.st->commit_pending)) return;
+  }
+  __shared__ double errsq[kBlock];
+  __shared__ double gs[2 * KT];
+  lr_reduce<R, KT>(part_in, nparts, K, w, gs);
+  double g[2 * KT];
+#pragma unroll
+  for (int q = 0; q < 2 * KT; ++q) g[q] = gs[q];
+  double acc[2 * KT];
+#pragma unroll
+  for (int q = 0; q < 2 * KT; ++q) acc[q] = 0.0;
+  const R h = (KIND >= 2) ? v.st->h : (R)0;
+  double sq = 0.0;
+  for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < v.N; i += (long)gridDim.x * kBlock) {
+    double x[KT];
+    double C = 0, S = 0;
+#pragma unroll
+    for (int m = 0; m < KT; ++m)
+      if (m < K) {
+        x[m] = X[i * K + m];
+        C = __builtin_fma(x[m], g[2 * m], C);
+        S = __builtin_fma(x[m], g[2 * m + 1], S);
+      }
+    sq += row_epilogue<R, KIND>(v, i, (R)C, (R)S, h);
+    if constexpr (NEXT) {
+      const double c = v.cs_out[i], s = v.cs_out[v.ldv + i];
+#pragma unroll
+      for (int m = 0; m < KT; ++m)
+        if (m < K) {
+          acc[2 * m] = __builtin_fma(x[m], c, acc[2 * m]);
+          acc[2 * m + 1] = __builtin_fma(x[m], s, acc[2 * m + 1]);
+        }
+    }
+  }
+  if constexpr (KIND == 7) {
+    const double tot = block_sum(sq, errsq);
+    if (threadIdx.x == 0) v.err_part[blockIdx.x] = tot;
+  }
+  if constexpr (NEXT) lr_write_partials<KT>(acc, K, part_out);
 }
 
 // ---------------------------------------------------------------------

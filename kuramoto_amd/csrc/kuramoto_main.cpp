@@ -22,6 +22,10 @@
 //                       prefactor) instead of the volcano matrix
 //   --adj-dtype=T       HBM storage of the dense adjacency: native (default),
 //                       bf16 or f16 (FP32 program only; FP32 accumulation)
+//   --rebuild           with -A: rebuild the adjacency tiles from the factor in
+//                       registers at every stage (dense semantics, O(N^2 K)
+//                       work, no N^2 storage) instead of the O(N K) factored
+//                       contraction
 //   --max-steps=n       stop after n accepted steps, leaving the files as a
 //                       process killed at that point would (no final block)
 //   --chunk=n           maximum step attempts per host synchronisation
@@ -90,7 +94,7 @@ static void usage() {
   std::printf("atol is absolute error tolerance. Default 1e-6.\n");
   std::printf("gpu is index of the gpu. Default 0.\n");
   std::printf("filebase is base file name for output. \n");
-  std::printf("extensions: --gauss --adj-dtype=native|bf16|f16 --max-steps=n --chunk=n\n");
+  std::printf("extensions: --gauss --adj-dtype=native|bf16|f16 --rebuild --max-steps=n --chunk=n\n");
 }
 
 static void append(const char *suffix, const void *data, size_t elem, size_t n) {
@@ -117,13 +121,14 @@ int main(int argc, char *argv[]) {
   const double start = now_s();
   R c1 = 3.0, t1 = 1e2, dt = 1e0, freq = 1.0, atl = 1e-6, rtl = 0, I = 0;
   int gpu = 0, seed = 1, fixed = 0, A = 0, verbose = 0, help = 1, dense = 3, normal = 0, reload = 0;
-  int gauss = 0, storage = KMT_ADJ_NATIVE, max_steps = -1, chunk_max = 64;
+  int gauss = 0, storage = KMT_ADJ_NATIVE, max_steps = -1, chunk_max = 64, rebuild = 0;
   unsigned long N = 5000, K = 6;
   const char *filebase = nullptr;
   static struct option longopts[] = {{"gauss", no_argument, nullptr, 1001},
                                      {"adj-dtype", required_argument, nullptr, 1002},
                                      {"max-steps", required_argument, nullptr, 1003},
                                      {"chunk", required_argument, nullptr, 1004},
+                                     {"rebuild", no_argument, nullptr, 1005},
                                      {nullptr, 0, nullptr, 0}};
   // kuramoto.cu:298-362 (options may follow the positional filebase)
   while (optind < argc) {
@@ -157,6 +162,7 @@ int main(int argc, char *argv[]) {
           break;
         case 1003: max_steps = atoi(optarg); break;
         case 1004: chunk_max = std::max(1, atoi(optarg)); break;
+        case 1005: rebuild = 1; break;
         default: break;
       }
     } else {
@@ -312,7 +318,7 @@ int main(int argc, char *argv[]) {
       }
     }
   }
-  if (A) CALL(kmt_use_factored(g_ctx, 1));
+  if (A) CALL(kmt_use_factored(g_ctx, rebuild ? KMT_LOWRANK_REBUILD : KMT_LOWRANK_CONTRACT));
   std::fflush(out);
   std::fclose(out);
 

@@ -146,3 +146,27 @@ def test_cli_lowrank_and_reload(tmp_path, km, orc):
     rr = load(r, "order.dat", np.float64)
     assert len(rr) == len(ro) and np.max(np.abs(rr - ro)) < 1e-3
     assert "Using adjacency from file" in open(r + ".out").read()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_cli_rebuild_equals_dense(tmp_path, km, orc, prec):
+    """`-A --rebuild` (adjacency tiles rebuilt in registers from the factor,
+    no adj.dat) gives the dense run's files bit for bit, and follows the
+    oracle's rebuilt-row run (`--rebuild`) to the order-parameter
+    tolerance."""
+    prog = os.path.join(BIN, "kuramoto" if prec == "f32" else "kuramoto_64")
+    oprog = os.path.join(ORC, "kuramoto_oracle" if prec == "f32" else "kuramoto_oracle_64")
+    dt = np.float32 if prec == "f32" else np.float64
+    g, d, o = str(tmp_path / "g"), str(tmp_path / "d"), str(tmp_path / "o")
+    args = ARGS + ["-D", "2", "-a", "1e-6" if prec == "f32" else "1e-9"]
+    print(f"case cli rebuild {prec}", flush=True)
+    run(prog, args + ["-A", "--rebuild"], g)
+    run(prog, args, d)
+    run(oprog, args + ["--rebuild"], o)
+    assert not os.path.exists(g + "adj.dat")
+    check_layout(g, 120, N_OUT, dt, dense=2, A=True)
+    for suf in ("order.dat", "thetas.dat", "couplings.dat", "times.dat", "fs.dat"):
+        assert open(g + suf, "rb").read() == open(d + suf, "rb").read(), suf
+    tol = 1e-6 if prec == "f32" else 1e-9
+    assert np.max(np.abs(load(g, "order.dat", dt) - load(o, "order.dat", dt))) < tol

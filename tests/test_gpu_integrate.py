@@ -158,12 +158,15 @@ def test_fixed_step_matches_oracle(km, orc):
 
 
 @pytest.mark.parametrize("prec", ["f32", "f64"])
-def test_factored_integration_matches_dense(km, orc, prec):
+@pytest.mark.parametrize("K", [5, 12, 20])
+def test_factored_integration_matches_dense(km, orc, prec, K):
     """-A (factored) and the dense matrix built from the same factor follow
     the same trajectory (reference: kuramoto vs kuramoto -A), checked
-    against the factored oracle (FP64: 1e-10; FP32: _fp32_ok)."""
+    against the factored oracle (FP64: 1e-10; FP32: _fp32_ok).  K = 5 and
+    12 run the one-launch-per-stage low-rank kernels (rank tiles of 8 and
+    16), K = 20 the three-launch path for larger ranks."""
     p = km.F32 if prec == "f32" else km.F64
-    N, K, seed, c1, t1 = 1500, 5, 3, 3.0, 1.0
+    N, seed, c1, t1 = 1500, 3, 3.0, 1.0
     atol = 1e-6 if prec == "f32" else 1e-9
     dtype, theta0, omega, X, w = _inputs(km, p, N, K, seed, normal=False, init=0.0)
     te = _grid(0.0, t1, 0.1, dtype)
@@ -249,3 +252,37 @@ def test_step_and_eval_api(km, orc):
     assert abs(ref["t"] - t) < 1e-9 * abs(t)
     assert np.max(np.abs(ref["y"] - y)) < 1e-8
     assert np.all(np.isfinite(mid))
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("K", [5, 12])
+def test_rebuilt_tiles_equal_dense(km, prec, K):
+    """-A with adjacency tiles rebuilt in registers (kmt_use_factored mode
+    KMT_LOWRANK_REBUILD) forms every M_ij with the dense generator's formula
+    and sums columns in the dense stage kernel's order: the RHS, the
+    couplings and a whole integration equal the dense path on the generated
+    matrix bit for bit."""
+    p = km.F32 if prec == "f32" else km.F64
+    N, seed, c1, t1 = 1337, 4, 3.0, 1.0
+    atol = 1e-6 if prec == "f32" else 1e-9
+    dtype, theta0, omega, X, w = _inputs(km, p, N, K, seed, normal=True, init=6.283185307)
+    te = _grid(0.0, t1, 0.25, dtype)
+    print(f"case rebuild K={K} prec={prec}", flush=True)
+    out = {}
+    for mode in ("dense", "rebuild"):
+        with km.Integrator(N, p, atol=atol, c1=c1) as it:
+            it.set_omega(omega)
+            it.set_factor(X, w)
+            if mode == "dense":
+                it.build_adjacency()
+            else:
+                it.use_factored("rebuild")
+            f, cpl = it.rhs(theta0, couplings=True)
+            it.set_state(theta0, 0.0, 0.01)
+            res = it.run(te, t1, keep_rows=True)
+            out[mode] = (f, cpl, res, it.get_state())
+    (fd, cd, rd, sd), (fr, cr, rr, sr) = out["dense"], out["rebuild"]
+    assert np.array_equal(fd, fr) and np.array_equal(cd, cr)
+    assert rd["status"].accepted == rr["status"].accepted and rd["status"].rejected == rr["status"].rejected
+    assert np.array_equal(rd["rows"], rr["rows"]) and np.array_equal(rd["r"], rr["r"])
+    assert np.array_equal(sd[0], sr[0]) and sd[1] == sr[1]
