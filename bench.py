@@ -31,40 +31,9 @@ sys.path.insert(0, ROOT)
 
 import numpy as np  # noqa: E402
 
-CONFIGS = {
-    # the FP64 plain path, the memory-bound gemv-roofline case
-    "fp64-n50000": dict(N=50000, K=5, seed=2, prec="f64", normal=True, init=6.283185307, c1=3.0, atol=1e-9,
-                        t1=100.0, dt=1.0, storage="native"),
-    # the north-star roofline target: N=50000 FP32 dense
-    "fp32-n50000": dict(N=50000, K=5, seed=2, prec="f32", normal=True, init=6.283185307, c1=3.0, atol=1e-6,
-                        t1=100.0, dt=1.0, storage="native"),
-    # the reference-defaults FP32 case (MALL-resident A)
-    "fp32-n5000": dict(N=5000, K=5, seed=1, prec="f32", normal=False, init=0.0, c1=3.0, atol=1e-6, t1=100.0,
-                       dt=1.0, storage="native"),
-    # -A, the low-rank factor only (N x K), coupled through the factor
-    "fp32-n200000-lowrank": dict(N=200000, K=5, seed=3, prec="f32", normal=False, init=0.0, c1=3.0, atol=1e-6,
-                                 t1=20.0, dt=0.5, storage="factored"),
-    # configs[2] in the north_star's literal form: adjacency tiles rebuilt in
-    # registers from X at every stage (dense semantics, VALU-bound)
-    "fp32-n200000-rebuild": dict(N=200000, K=5, seed=3, prec="f32", normal=False, init=0.0, c1=3.0, atol=1e-6,
-                                 t1=20.0, dt=0.5, storage="rebuild"),
-    # configs[2]'s check run: the same A stored explicitly in FP16 (80 GB)
-    "fp32-n200000-f16": dict(N=200000, K=5, seed=3, prec="f32", normal=False, init=0.0, c1=3.0, atol=1e-6,
-                             t1=20.0, dt=0.5, storage="f16"),
-}
-
-# batched configurations (FP32 state, 16-bit adjacency)
-BATCH_CONFIGS = {
-    # B=64 sharing one N=20000 BF16 adjacency, c_b = linspace(0.5, 4, 64),
-    # Cauchy frequencies, theta0 = pi(2u-1), seeds 1..64, t=50
-    "batch64-n20000-bf16": dict(B=64, Ns=[20000] * 64, shared=True, seeds=list(range(1, 65)),
-                                cs=list(np.linspace(0.5, 4.0, 64)), normal=[False] * 64, t1=50.0, dt=1.0),
-    # B=48, N uniform in {5000..40000}, own adjacency each (seeds
-    # 100..147), normal/Cauchy alternating, c=3, t=100, with the learned
-    # step proposer against the same batch without it
-    "batch48-uneven-bf16": dict(B=48, Ns=None, shared=False, seeds=list(range(100, 148)), cs=[3.0] * 48,
-                                normal=[b % 2 == 0 for b in range(48)], t1=100.0, dt=1.0),
-}
+from kuramoto_amd.workloads import BATCH as BATCH_CONFIGS  # noqa: E402
+from kuramoto_amd.workloads import PLAIN as CONFIGS  # noqa: E402
+from kuramoto_amd.workloads import make_batch  # noqa: E402
 
 HBM_PEAK = 8.0e12
 
@@ -147,35 +116,6 @@ def cpu_baseline(cfg, max_steps):
                        f"first {res['accepted']} accepted steps "
                        f"({res['rejected']} rejected, {res['rhs_evals']} RHS evals) in {secs:.1f} s, "
                        f"OpenMP threads={orc.num_threads()}")
-
-
-def uneven_sizes(B, seed=147):
-    """N_b drawn uniformly from {5000, 10000, ..., 40000} (Philox-free, fixed)."""
-    rng = np.random.default_rng(seed)
-    return [int(5000 * rng.integers(1, 9)) for _ in range(B)]
-
-
-def make_batch(name, proposer):
-    import kuramoto_amd as km
-
-    cfg = BATCH_CONFIGS[name]
-    B = cfg["B"]
-    Ns = cfg["Ns"] or uneven_sizes(B)
-    bt = km.Batch(Ns, atol=1e-6, shared=cfg["shared"], storage="bf16")
-    if cfg["shared"]:
-        X = km.factor(Ns[0], 5, 1, gaussian=True).astype(np.float32)
-        bt.set_factor(-1, X, km.factor_weights(km.F32, Ns[0], 5, gaussian=True))
-    for b in range(B):
-        th, om = km.draws(km.F32, Ns[b], cfg["seeds"][b], normal=cfg["normal"][b], init=6.283185307)
-        bt.set_instance(b, om, th, float(cfg["cs"][b]), 0.0, 0.01)
-        if not cfg["shared"]:
-            X = km.factor(Ns[b], 5, cfg["seeds"][b], gaussian=True).astype(np.float32)
-            bt.set_factor(b, X, km.factor_weights(km.F32, Ns[b], 5, gaussian=True))
-    tune = bt.autotune(3) if cfg["shared"] else None
-    if proposer:
-        w = np.load(os.path.join(ROOT, "kuramoto_amd", "proposer_weights.npz"))["w16"]
-        bt.set_proposer(w, True)
-    return bt, Ns, tune
 
 
 def batch_to_time(name, proposer, t_end):
@@ -292,8 +232,8 @@ def main():
                     help="uneven batch: end time of the whole runs that compare the two step controllers")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (see DESIGN.md, "
-              "'How the path would shard')", file=sys.stderr)
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (the row-sharded "
+              "multi-GPU layout is a design only)", file=sys.stderr)
         sys.exit(2)
     if args.config in BATCH_CONFIGS:
         batch_main(args)

@@ -187,7 +187,7 @@ int kmt_autotune(kmt_ctx *ctx, int reps, double *variant_ms);
  * of the stage kernel (negative when the non-temporal load variant is off). */
 int kmt_layout(kmt_ctx *ctx, long *lda, int *elem_bytes, int *rows_per_block);
 
-/* ---- batched integration (north_star extension; no reference function:
+/* ---- batched integration (an extension; no reference function:
  * the reference runs one instance per process, SLURM arrays of
  * kuramoto calls, job1.sh:13-25) ------------------------------------------
  * B independent FP32 runs advanced in lockstep stages on one GPU, each with

@@ -484,7 +484,7 @@ __device__ float mlp_eval(const uint16_t *w, const float *x) {
 
 __device__ void proposer_features(const BInst *in, float h_std, float *x) {
   // last 3 log h (relative to the standard proposal), last 3 log error ratios,
-  // r and dr/dt (the north_star's inputs)
+  // r and dr/dt
   const float lh = logf(h_std);
   for (int q = 0; q < 3; ++q) x[q] = in->hist_n > q ? in->logh[q] - lh : 0.f;
   for (int q = 0; q < 3; ++q) x[3 + q] = in->hist_n > q ? in->logerr[q] : 0.f;

@@ -1,7 +1,7 @@
 // Learned step-size proposer: training and inference kernels of the MLP
 // 8 -> 64 -> 64 -> 1 (tanh, tanh, linear) used by the batched controller.
 // FP16 weights for inference, FP32 master weights for the optimizer, FP32
-// accumulation everywhere.  No reference counterpart (north_star extension).
+// accumulation everywhere.  No reference counterpart (an extension of the path).
 //
 // Parameter packing (4801 values): W1[64][8], b1[64], W2[64][64], b2[64],
 // W3[64], b3.  Loss: mean squared error (1/S) sum_s (out_s - y_s)^2.

@@ -124,7 +124,7 @@ def test_cli_kill_resume_bit_exact(tmp_path, km):
 def test_cli_lowrank_and_reload(tmp_path, km, orc):
     """-A (factored coupling, no adj.dat) against the oracle's -A; then -R
     with the oracle's frequency and adjacency files gives the oracle's
-    trajectory (identical inputs through -R, as the north_star asks)."""
+    trajectory (identical inputs through -R)."""
     g, o = str(tmp_path / "g"), str(tmp_path / "o")
     args = ARGS + ["-A", "-a", "1e-9", "-D", "2"]
     print("case cli -A", flush=True)
@@ -170,3 +170,57 @@ def test_cli_rebuild_equals_dense(tmp_path, km, orc, prec):
         assert open(g + suf, "rb").read() == open(d + suf, "rb").read(), suf
     tol = 1e-6 if prec == "f32" else 1e-9
     assert np.max(np.abs(load(g, "order.dat", dt) - load(o, "order.dat", dt))) < tol
+
+
+def _dmd_load(filebase0):
+    """The reading logic of dmd.py's load_data (dmd.py:16-60), restated with
+    numpy: for n = 0, 1, ... while FILEBASE<n>.out exists, parse the first
+    .out line as float64 (N K t1 dt c seed), take length = int(t1/dt) rows of
+    float64 thetas.dat (shape (length, N)) and the float64 frequencies.dat,
+    and move to the frame rotating at the mean frequency."""
+    out, n = [], 0
+    while os.path.exists(f"{filebase0}{n}.out"):
+        fb = f"{filebase0}{n}"
+        N, K, t1, dt, c, seed = np.array(open(fb + ".out").readline().split(), dtype=np.float64)
+        N, length = int(N), int(t1 / dt)
+        omega = np.memmap(fb + "frequencies.dat", dtype=np.float64, mode="r+", shape=(N,))
+        theta = np.memmap(fb + "thetas.dat", dtype=np.float64, mode="r+", shape=(length, N))
+        out.append((np.array(theta) - np.mean(omega) * dt * np.arange(length)[:, None], (N, K, t1, dt, c, seed)))
+        n += 1
+    return out
+
+
+@pytest.mark.gpu
+def test_dmd_load_data_reads_fp64_outputs(tmp_path, km, orc):
+    """Downstream format consumer: two kuramoto_64 trajectories (FILEBASE0,
+    FILEBASE1) read through dmd.py's load_data logic give the header values,
+    (t1/dt) x N float64 theta rows starting with the initial phases, and the
+    same rotating-frame data as the oracle's files read the same way."""
+    g0, o0 = str(tmp_path / "g"), str(tmp_path / "o")
+    for n, seed in enumerate((3, 4)):
+        args = ["-N", "96", "-K", "5", "-c", "2.5", "-t", "2", "-d", "0.25", "-s", str(seed), "-n", "-a", "1e-9",
+                "-I", "6.283185307"]
+        print(f"case dmd trajectory {n}", flush=True)
+        run(os.path.join(BIN, "kuramoto_64"), args, f"{g0}{n}")
+        run(os.path.join(ORC, "kuramoto_oracle_64"), args, f"{o0}{n}")
+    gl, ol = _dmd_load(g0), _dmd_load(o0)
+    assert len(gl) == len(ol) == 2
+    for (tg, hg), (to, ho) in zip(gl, ol):
+        assert hg == ho and hg[0] == 96 and tg.shape == (8, 96)
+        assert np.max(np.abs(tg - to)) < 1e-8
+    assert np.array_equal(gl[0][0][0], np.fromfile(f"{g0}0thetas.dat", dtype=np.float64)[:96])
+
+
+def test_dmd_load_data_on_oracle_outputs(tmp_path, orc):
+    """dmd.py's reading logic on the oracle CLI's FP64 files (CPU): header,
+    (t1/dt) rows of N, row 0 = initial phases, rotating frame."""
+    o0 = str(tmp_path / "o")
+    args = ["-N", "64", "-K", "5", "-c", "2.5", "-t", "1", "-d", "0.125", "-s", "5", "-n", "-a", "1e-9",
+            "-I", "6.283185307"]
+    run(os.path.join(ORC, "kuramoto_oracle_64"), args, f"{o0}0")
+    (theta, hdr), = _dmd_load(o0)
+    assert hdr == (64.0, 5.0, 1.0, 0.125, 2.5, 5.0) and theta.shape == (8, 64)
+    raw = np.fromfile(f"{o0}0thetas.dat", dtype=np.float64).reshape(-1, 64)
+    om = np.fromfile(f"{o0}0frequencies.dat", dtype=np.float64)
+    assert raw.shape[0] == 9 and np.array_equal(theta[0], raw[0])
+    assert np.allclose(theta[3], raw[3] - om.mean() * 0.125 * 3, rtol=0, atol=1e-12)

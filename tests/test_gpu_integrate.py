@@ -6,10 +6,10 @@ output grid and order parameter (kuramoto.cu:164-249), main's final-state
 interpolation (kuramoto.cu:630-647).
 
 Tolerances: FP64 phases at t=1 within 1e-10 of the FP64 oracle on the same
-inputs (north_star); FP32 phases held to the accuracy of the reference's own
-FP32 method (see _fp32_ok: at atol=1e-6 the FP32 oracle is itself 5e-6..2e-4
-from FP64, so the north_star's 1e-5 is out of reach of the method, not of
-the kernels); r(t) within 1e-3; accepted-step counts within 2% (or 1).  The
+inputs; FP32 phases held to the accuracy of the reference's own FP32 method
+(see _fp32_ok: at atol=1e-6 the FP32 oracle is itself 5e-6..2e-4 from FP64,
+so a fixed 1e-5 bound is out of reach of the method, not of the kernels);
+r(t) within 1e-3; accepted-step counts within 2% (or 1).  The
 K=0 controller KAT is bit-exact in t and step count.
 """
 import json

@@ -121,7 +121,8 @@ def test_rhs_brute_force(orc, N):
 
 def test_rhs_sign_is_pinned(orc):
     """Adler pair: phi' = dw + 2 c a sin(phi).  A flipped sign (the
-    north_star's formula without the generator's sign flip) fails this."""
+    textbook c/N sin(theta_j - theta_i) form without the generator's sign
+    flip) fails this."""
     a, c1 = 0.4, 2.0
     M = np.array([[0.0, a], [a, 0.0]])
     om = np.array([0.3, -0.5])
