@@ -72,8 +72,8 @@ def compare(r, final, ref):
 
 def plain(name, t1=None, storage=None):
     cfg = dict(workloads.PLAIN[name])
-    if cfg["prec"] != "f32" or cfg["storage"] not in ("native", "bf16", "f16"):
-        raise ValueError(f"{name}: the check covers FP32 dense workloads")
+    if cfg["prec"] != "f32":
+        raise ValueError(f"{name}: the check covers FP32 workloads")
     if storage:
         cfg["storage"] = storage
     N, K = cfg["N"], cfg["K"]
@@ -86,8 +86,13 @@ def plain(name, t1=None, storage=None):
     with km.Integrator(N, km.F32, atol=cfg["atol"], c1=cfg["c1"]) as it:
         it.set_omega(om)
         it.set_factor(X.astype(np.float32), w32)
-        it.build_adjacency(cfg["storage"])
-        same = cfg["storage"] != "native" and 8.0 * N * N <= DENSE64_LIMIT
+        if cfg["storage"] == "factored":
+            it.use_factored("contract")
+        elif cfg["storage"] == "rebuild":
+            it.use_factored("rebuild")
+        else:
+            it.build_adjacency(cfg["storage"])
+        same = cfg["storage"] in ("bf16", "f16") and 8.0 * N * N <= DENSE64_LIMIT
         Mq = it.adjacency_rows(0, N).astype(np.float64) if same else None
         it.set_state(th, 0.0, h0)
         res = it.run(_grid(t1, cfg["dt"]), t1, keep_rows=False, ring_rows=256, chunk=64)
