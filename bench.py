@@ -81,11 +81,11 @@ def gpu_run(cfg, steps, warmup):
     it.read_outputs()
     st0 = it.advance(0)
     region_ms, stage_ms, st = it.time_attempts(steps)
-    lda, eb, rows = it.layout()
+    lda, eb, rows, nt, bal = it.layout()
     it.close()
     copy_gbps, read_gbps = km.bandwidth_probe(8 << 30, 5)
     return dict(dtype=dtype, N=N, region_ms=region_ms, stage_ms=stage_ms, st0=st0, st=st, lda=lda, elem_bytes=eb,
-                gen_s=gen_s, rows=abs(rows), nt=rows > 0, copy_gbps=copy_gbps, read_gbps=read_gbps)
+                gen_s=gen_s, rows=rows, nt=nt, balanced=bal, copy_gbps=copy_gbps, read_gbps=read_gbps)
 
 
 def cpu_baseline(cfg, max_steps):
@@ -289,7 +289,8 @@ def main():
                                 else "rebuild_stage_kernel (tiles rebuilt in registers)" if cfg["storage"] == "rebuild"
                                 else "stage_kernel (fused gemv-pair + DP45 stage epilogue)"),
                      "bytes_per_launch": alg_bytes, "avg_launch_ms": r["stage_ms"],
-                     "variant": {"rows_per_workgroup": r["rows"], "nontemporal_loads": r["nt"]},
+                     "variant": {"rows_per_group": r["rows"], "nontemporal_loads": r["nt"],
+                                 "balanced_grid": r["balanced"]},
                      "measured_read_stream_gbps": r["read_gbps"], "measured_copy_gbps": r["copy_gbps"],
                      "frac_of_measured_read_stream": achieved / 1e9 / r["read_gbps"]},
     }

@@ -174,18 +174,24 @@ int kmt_time_attempts(kmt_ctx *ctx, int attempts, double *region_ms, double *sta
 int kmt_bandwidth_probe(int device, size_t bytes, int reps, double *copy_gbps, double *read_gbps);
 
 /* Autotune the dense stage kernel on this context's adjacency: rows per
- * workgroup {4, 8, 16} x non-temporal loads {off, on}, `reps` timed launches
- * each; keeps the fastest.  variant_ms (6 values, may be NULL) receives the
- * average ms of each variant in the order (4,off) (4,on) (8,off) (8,on)
- * (16,off) (16,on).  kmt_begin runs it once by itself for N >= 4096.  The
- * variants differ in speed only: every row is summed in the same order.  No
- * reference counterpart (cuBLAS chooses its own gemv kernel). */
+ * workgroup {4, 8, 16} x non-temporal loads {off, on} x grid {one workgroup
+ * per row group, balanced (one resident round, equal contiguous row
+ * ranges)}, `reps` timed launches each; keeps the fastest.  variant_ms (12
+ * values, may be NULL) receives the average ms of each variant, index
+ * 4*r + 2*nt + balanced with r = 0, 1, 2 for 4, 8, 16 rows.  kmt_begin runs
+ * it once by itself for N >= 4096.  The variants differ in speed only: every
+ * row is summed in the same order.  No reference counterpart (cuBLAS
+ * chooses its own gemv kernel). */
 int kmt_autotune(kmt_ctx *ctx, int reps, double *variant_ms);
+/* Pin the dense stage-kernel variant instead of the autotuner's choice:
+ * rows per group 4, 8 or 16; variant_flags as kmt_layout reports them. */
+int kmt_select_variant(kmt_ctx *ctx, int rows_per_block, int variant_flags);
 
 /* Sizes the library chose (for roofline bookkeeping): padded row stride of
- * the adjacency in elements, bytes per stored element, rows per workgroup
- * of the stage kernel (negative when the non-temporal load variant is off). */
-int kmt_layout(kmt_ctx *ctx, long *lda, int *elem_bytes, int *rows_per_block);
+ * the adjacency in elements, bytes per stored element, rows per group of the
+ * stage kernel, and variant_flags (bit 0: non-temporal adjacency loads, bit
+ * 1: balanced grid).  Any pointer may be NULL. */
+int kmt_layout(kmt_ctx *ctx, long *lda, int *elem_bytes, int *rows_per_block, int *variant_flags);
 
 /* ---- batched integration (an extension; no reference function:
  * the reference runs one instance per process, SLURM arrays of

@@ -88,8 +88,9 @@ _SIGNATURES = {
     "kmt_eval": ([_P, _D, _P], _I),
     "kmt_time_stage_kernel": ([_P, _I, _P], _I),
     "kmt_time_attempts": ([_P, _I, _P, _P, _P], _I),
-    "kmt_layout": ([_P, _P, _P, _P], _I),
+    "kmt_layout": ([_P, _P, _P, _P, _P], _I),
     "kmt_autotune": ([_P, _I, _P], _I),
+    "kmt_select_variant": ([_P, _I, _I], _I),
     "kmt_batch_create": ([_P, _I, _I, _P, _D, _D, _I, _I, _I], _I),
     "kmt_batch_destroy": ([_P], _I),
     "kmt_batch_set_instance": ([_P, _I, _P, _P, _D, _D, _D], _I),
@@ -339,14 +340,22 @@ class Integrator:
         return region.value, stage.value, st
 
     def autotune(self, reps=3):
-        ms = np.zeros(6)
+        """Times the 12 stage-kernel variants; returns their ms as an array
+        indexed [rows 4/8/16][non-temporal off/on][balanced grid off/on]."""
+        ms = np.zeros(12)
         _check(self.lib.kmt_autotune(self.ctx, reps, _ptr(ms)), "kmt_autotune")
-        return ms
+        return ms.reshape(3, 2, 2)
+
+    def select_variant(self, rows, nontemporal=True, balanced=False):
+        _check(self.lib.kmt_select_variant(self.ctx, rows, int(nontemporal) | (2 if balanced else 0)),
+               "kmt_select_variant")
 
     def layout(self):
-        lda, eb, rows = ctypes.c_long(), ctypes.c_int(), ctypes.c_int()
-        _check(self.lib.kmt_layout(self.ctx, ctypes.byref(lda), ctypes.byref(eb), ctypes.byref(rows)), "kmt_layout")
-        return lda.value, eb.value, rows.value
+        """(lda, bytes per stored element, rows per group, non-temporal, balanced)."""
+        lda, eb, rows, fl = ctypes.c_long(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(self.lib.kmt_layout(self.ctx, ctypes.byref(lda), ctypes.byref(eb), ctypes.byref(rows),
+                                   ctypes.byref(fl)), "kmt_layout")
+        return lda.value, eb.value, rows.value, bool(fl.value & 1), bool(fl.value & 2)
 
     def run(self, t_eval, t1, keep_rows=True, ring_rows=64, chunk=32, max_attempts=None):
         """dp45_run + step_eval's output walk: returns a dict with the phase

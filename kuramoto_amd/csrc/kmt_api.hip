@@ -104,8 +104,9 @@ struct EngineBase {
   virtual int eval(double, void *) = 0;
   virtual int time_stage(int, double *) = 0;
   virtual int time_attempts(int, double *, double *, kmt_status *) = 0;
-  virtual int layout(long *, int *, int *) = 0;
+  virtual int layout(long *, int *, int *, int *) = 0;
   virtual int tune(int, double *) = 0;
+  virtual int select_variant(int, int) = 0;
   double mscale_ = 1.0;
 };
 
@@ -148,8 +149,9 @@ template <class R> struct Engine : EngineBase {
   int ring_cap = 0, keep_rows = 0, times_cap = 4096, nblk_emit = 0;
   hipGraphExec_t g1 = nullptr, g8 = nullptr;
   bool graphs_valid = false;
-  int tune_rows = 8, tune_nt = 1, tuned = 0;
-  double tune_ms[6] = {0, 0, 0, 0, 0, 0};
+  int tune_rows = 8, tune_nt = 1, tune_bal = 0, tuned = 0;
+  double tune_ms[12] = {0};
+  int bal_grid = 0;  // workgroups of the balanced variant (one resident round)
   R t0_ = 0, h0_ = 0;
 
   ~Engine() override { release(); }
@@ -208,7 +210,9 @@ template <class R> struct Engine : EngineBase {
     KMT_TRY(alloc_vec(&vec_scr, ldv));
     KMT_TRY(alloc_vec(&f_scr, ldv));
     KMT_TRY(alloc_vec(&pairs, 2 * ldv));
-    n_err_part = (int)std::max<long>(rows_alloc / 4, (ldv + kBlock - 1) / kBlock);
+    // one error partial per stage-7 workgroup: at most one per row (every
+    // grid - row groups, balanced, low-rank, rebuilt rows - has <= N workgroups)
+    n_err_part = (int)rows_alloc;
     HIP_TRY(hipMalloc((void **)&err_part, sizeof(double) * n_err_part));
     HIP_TRY(hipMalloc((void **)&ord_scr, sizeof(double) * (2 * ((ldv + kBlock - 1) / kBlock) + 2)));
     HIP_TRY(hipMalloc((void **)&st, sizeof(State<R>)));
@@ -223,15 +227,27 @@ template <class R> struct Engine : EngineBase {
     return KMT_OK;
   }
 
-  int layout(long *plda, int *eb, int *rows) override {
+  int layout(long *plda, int *eb, int *rows, int *flags) override {
     if (plda) *plda = lda;
     if (eb) *eb = storage == KMT_ADJ_BF16 || storage == KMT_ADJ_F16 ? 2 : (int)sizeof(R);
-    if (rows) *rows = tune_nt ? tune_rows : -tune_rows;
+    if (rows) *rows = tune_rows;
+    if (flags) *flags = (tune_nt ? 1 : 0) | (tune_bal ? 2 : 0);
     return KMT_OK;
   }
-  int tune(int reps, double *ms6) override {
+  int select_variant(int rows, int flags) override {
+    if (rows != 4 && rows != 8 && rows != 16) return fail(KMT_ERR_ARG, "rows per group must be 4, 8 or 16 (got %d)", rows);
+    if (flags & ~3) return fail(KMT_ERR_ARG, "unknown variant flags 0x%x", flags);
+    tune_rows = rows;
+    tune_nt = flags & 1;
+    tune_bal = (flags >> 1) & 1;
+    bal_grid = 0;
+    tuned = 1;
+    drop_graphs();
+    return KMT_OK;
+  }
+  int tune(int reps, double *ms12) override {
     KMT_TRY(autotune(reps));
-    if (ms6) for (int q = 0; q < 6; ++q) ms6[q] = tune_ms[q];
+    if (ms12) for (int q = 0; q < 12; ++q) ms12[q] = tune_ms[q];
     return KMT_OK;
   }
 
@@ -305,6 +321,7 @@ template <class R> struct Engine : EngineBase {
     A_bytes = bytes;
     storage = stg;
     tuned = 0;
+    bal_grid = 0;
     HIP_TRY(hipMemsetAsync(A, 0, bytes, stream));
     return KMT_OK;
   }
@@ -503,9 +520,24 @@ template <class R> struct Engine : EngineBase {
 
   int coupling_ready() const { return factored ? (X != nullptr) : (A != nullptr); }
 
+  // workgroups of one resident round of the balanced variant (<= N rows)
+  template <class S, int ROWS, bool NT> int balanced_grid() {
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, stage_kernel<R, S, ROWS, 3, NT, true>, kBlock, 0) !=
+            hipSuccess || per_cu < 1)
+      per_cu = 1;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus < 1) cus = 1;
+    return (int)std::min<long>(std::min<long>((long)per_cu * cus, N), n_err_part);
+  }
+
   template <class S, int KIND, int ROWS, bool NT> void launch_dense_variant(const DenseArgs<R, S> &a) {
-    const unsigned grid = (unsigned)((N + ROWS - 1) / ROWS);
-    stage_kernel<R, S, ROWS, KIND, NT><<<grid, kBlock, 0, stream>>>(a);
+    if (tune_bal) {
+      if (bal_grid <= 0) bal_grid = balanced_grid<S, ROWS, NT>();
+      stage_kernel<R, S, ROWS, KIND, NT, true><<<(unsigned)bal_grid, kBlock, 0, stream>>>(a);
+    } else {
+      const unsigned grid = (unsigned)((N + ROWS - 1) / ROWS);
+      stage_kernel<R, S, ROWS, KIND, NT, false><<<grid, kBlock, 0, stream>>>(a);
+    }
   }
 
   template <class S, int KIND> void launch_dense_kind(const Vecs<R> &v) {
@@ -630,13 +662,16 @@ template <class R> struct Engine : EngineBase {
   int n_parts_stage7() const {
     if (factored == KMT_LOWRANK_REBUILD) return (int)((N + rebuild_rows() - 1) / rebuild_rows());
     if (factored) return lr_fused() ? lr_grid : (int)((N + kBlock - 1) / kBlock);
+    if (tune_bal) return bal_grid;
     return (int)((N + tune_rows - 1) / tune_rows);
   }
 
   // Autotuner of the dense stage kernel: rows per workgroup {4, 8, 16} x
-  // non-temporal adjacency loads {off, on}, each timed on this context's own
-  // matrix (stage 3 on scratch buffers).  Every variant sums each row in the
-  // same order, so the choice changes speed only, never results.
+  // non-temporal adjacency loads {off, on} x grid {one workgroup per row
+  // group, balanced: one resident round with contiguous equal row ranges},
+  // each timed on this context's own matrix (stage 3 on scratch buffers).
+  // Every variant sums each row in the same order, so the choice changes
+  // speed only, never results.
   int autotune(int reps) {
     if (factored || !A) return KMT_OK;
     if (!k_scr) {
@@ -650,11 +685,14 @@ template <class R> struct Engine : EngineBase {
     HIP_TRY(hipEventCreate(&e1));
     const int rows_opts[3] = {4, 8, 16};
     double best = 1e30;
-    int best_rows = 8, best_nt = 1;
+    int best_rows = 8, best_nt = 1, best_bal = 0, best_grid = 0;
     for (int ri = 0; ri < 3; ++ri)
-      for (int nt = 0; nt < 2; ++nt) {
+      for (int nt = 0; nt < 2; ++nt)
+      for (int bal = 0; bal < 2; ++bal) {
         tune_rows = rows_opts[ri];
         tune_nt = nt;
+        tune_bal = bal;
+        bal_grid = 0;
         launch_by_kind(3, v);
         hipEventRecord(e0, stream);
         for (int r = 0; r < reps; ++r) launch_by_kind(3, v);
@@ -662,13 +700,15 @@ template <class R> struct Engine : EngineBase {
         HIP_TRY(hipEventSynchronize(e1));
         float ms = 0;
         HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-        tune_ms[ri * 2 + nt] = ms / reps;
-        if (ms < best) { best = ms; best_rows = tune_rows; best_nt = nt; }
+        tune_ms[ri * 4 + nt * 2 + bal] = ms / reps;
+        if (ms < best) { best = ms; best_rows = tune_rows; best_nt = nt; best_bal = bal; best_grid = bal_grid; }
       }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     tune_rows = best_rows;
     tune_nt = best_nt;
+    tune_bal = best_bal;
+    bal_grid = best_grid;
     tuned = 1;
     drop_graphs();
     return KMT_OK;
@@ -1201,9 +1241,13 @@ int kmt_time_attempts(kmt_ctx *ctx, int attempts, double *region_ms, double *sta
   CTX_CHECK;
   return ctx->eng->time_attempts(attempts, region_ms, stage_avg_ms, st);
 }
-int kmt_layout(kmt_ctx *ctx, long *lda, int *elem_bytes, int *rows_per_block) {
+int kmt_select_variant(kmt_ctx *ctx, int rows_per_block, int variant_flags) {
   CTX_CHECK;
-  return ctx->eng->layout(lda, elem_bytes, rows_per_block);
+  return ctx->eng->select_variant(rows_per_block, variant_flags);
+}
+int kmt_layout(kmt_ctx *ctx, long *lda, int *elem_bytes, int *rows_per_block, int *variant_flags) {
+  CTX_CHECK;
+  return ctx->eng->layout(lda, elem_bytes, rows_per_block, variant_flags);
 }
 
 int kmt_autotune(kmt_ctx *ctx, int reps, double *variant_ms) {

@@ -254,12 +254,12 @@ template <class R, class S> struct DenseArgs {
   Vecs<R> v;
 };
 
-// Row sums of a stage kernel: wave shuffles, then the 4 wave sums in wave
-// order (fixed tree), scale, row epilogue; stage 7 writes one error partial
-// per workgroup.
+// Row sums of one group of ROWS rows [row0, row_end): wave shuffles, then
+// the 4 wave sums in wave order (fixed tree), scale, row epilogue.  Returns
+// (in thread 0) the group's sum of squared scaled errors for stage 7.
 template <class R, class Acc, int ROWS, int KIND>
-__device__ __forceinline__ void finish_rows(Acc (&accC)[ROWS], Acc (&accS)[ROWS], long row0, R mscale,
-                                            const Vecs<R> &v) {
+__device__ __forceinline__ double finish_group(Acc (&accC)[ROWS], Acc (&accS)[ROWS], long row0, long row_end,
+                                               R mscale, const Vecs<R> &v) {
   __shared__ Acc red[kBlock / kWave][2 * ROWS];
   __shared__ double errsq[ROWS];
   const int tid = threadIdx.x;
@@ -277,7 +277,7 @@ __device__ __forceinline__ void finish_rows(Acc (&accC)[ROWS], Acc (&accS)[ROWS]
   if (tid < ROWS) {
     const long i = row0 + tid;
     double sq = 0.0;
-    if (i < v.N) {
+    if (i < row_end) {
       Acc C = red[0][tid], Sv = red[0][ROWS + tid];
 #pragma unroll
       for (int w = 1; w < kBlock / kWave; ++w) { C += red[w][tid]; Sv += red[w][ROWS + tid]; }
@@ -286,18 +286,36 @@ __device__ __forceinline__ void finish_rows(Acc (&accC)[ROWS], Acc (&accS)[ROWS]
     }
     if constexpr (KIND == 7) errsq[tid] = sq;
   }
+  double s = 0.0;
   if constexpr (KIND == 7) {
     __syncthreads();
     if (tid == 0) {
-      double s = 0.0;
 #pragma unroll
       for (int r = 0; r < ROWS; ++r) s += errsq[r];
-      v.err_part[blockIdx.x] = s;
     }
+  }
+  return s;
+}
+
+// One group per workgroup: rows [row0, min(row0 + ROWS, N)); stage 7 writes
+// one error partial per workgroup.
+template <class R, class Acc, int ROWS, int KIND>
+__device__ __forceinline__ void finish_rows(Acc (&accC)[ROWS], Acc (&accS)[ROWS], long row0, R mscale,
+                                            const Vecs<R> &v) {
+  const double s = finish_group<R, Acc, ROWS, KIND>(accC, accS, row0, v.N, mscale, v);
+  if constexpr (KIND == 7) {
+    if (threadIdx.x == 0) v.err_part[blockIdx.x] = s;
   }
 }
 
-template <class R, class S, int ROWS, int KIND, bool NT>
+// BAL = false: workgroup b owns rows [b ROWS, b ROWS + ROWS).
+// BAL = true ("balanced"): the grid is one round of resident workgroups and
+// workgroup b owns the contiguous rows [b N / G, (b + 1) N / G), walked in
+// groups of ROWS, so every workgroup streams the same number of rows (±1)
+// and the launch has no tail round.  Rows past the end of a short group
+// re-read the group's last row (cache hits) and are discarded.  Every row is
+// summed in the same order in every variant.
+template <class R, class S, int ROWS, int KIND, bool NT, bool BAL>
 __global__ __launch_bounds__(kBlock) void stage_kernel(const DenseArgs<R, S> a) {
   if constexpr (KIND >= 2) {
     if (a.v.check_state && (!a.v.st->active || a.v.st->commit_pending)) return;
@@ -306,69 +324,87 @@ __global__ __launch_bounds__(kBlock) void stage_kernel(const DenseArgs<R, S> a) 
   constexpr int VEC = Store<S>::VEC;
   constexpr long STRIDE = (long)kBlock * VEC;
   const int tid = threadIdx.x;
-  const long row0 = (long)blockIdx.x * ROWS;
-  const S *Ab = a.A + row0 * a.lda;
+  const long N = a.v.N;
+  long rb, re;
+  if constexpr (BAL) {
+    rb = (long)blockIdx.x * N / gridDim.x;
+    re = (long)(blockIdx.x + 1) * N / gridDim.x;
+  } else {
+    rb = (long)blockIdx.x * ROWS;
+    re = rb + ROWS < N ? rb + ROWS : N;
+  }
   const R *cosv = a.v.cs_in;
   const R *sinv = a.v.cs_in + a.v.ldv;
-
-  Acc accC[ROWS], accS[ROWS];
-#pragma unroll
-  for (int r = 0; r < ROWS; ++r) { accC[r] = 0; accS[r] = 0; }
-
   auto load_row = [&](const S *p) -> u32x4 {
     if constexpr (NT) return __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(p));
     else return *reinterpret_cast<const u32x4 *>(p);
   };
+  double esum = 0.0;
+  for (long row0 = rb; row0 < re; row0 += ROWS) {
+    const long nr = re - row0 < ROWS ? re - row0 : ROWS;
+    // rows past N of the last group (non-balanced grid) are zero padding rows
+    // of the allocation; the balanced grid re-reads the group's last row
+    const S *rowp[ROWS];
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) rowp[r] = a.A + (row0 + (BAL ? (r < nr ? r : nr - 1) : r)) * a.lda;
 
-  long c = (long)tid * VEC;
-  for (; c + STRIDE < a.lda; c += 2 * STRIDE) {
-    R cv0[VEC], sv0[VEC], cv1[VEC], sv1[VEC];
-    u32x4 raw0[ROWS], raw1[ROWS];
+    Acc accC[ROWS], accS[ROWS];
 #pragma unroll
-    for (int r = 0; r < ROWS; ++r) raw0[r] = load_row(Ab + r * a.lda + c);
+    for (int r = 0; r < ROWS; ++r) { accC[r] = 0; accS[r] = 0; }
+
+    long c = (long)tid * VEC;
+    for (; c + STRIDE < a.lda; c += 2 * STRIDE) {
+      R cv0[VEC], sv0[VEC], cv1[VEC], sv1[VEC];
+      u32x4 raw0[ROWS], raw1[ROWS];
 #pragma unroll
-    for (int r = 0; r < ROWS; ++r) raw1[r] = load_row(Ab + r * a.lda + c + STRIDE);
-    load_reals<R, VEC>(cosv + c, cv0);
-    load_reals<R, VEC>(sinv + c, sv0);
-    load_reals<R, VEC>(cosv + c + STRIDE, cv1);
-    load_reals<R, VEC>(sinv + c + STRIDE, sv1);
+      for (int r = 0; r < ROWS; ++r) raw0[r] = load_row(rowp[r] + c);
 #pragma unroll
-    for (int r = 0; r < ROWS; ++r) {
-      Acc m[VEC];
-      unpack<S>(raw0[r], m);
+      for (int r = 0; r < ROWS; ++r) raw1[r] = load_row(rowp[r] + c + STRIDE);
+      load_reals<R, VEC>(cosv + c, cv0);
+      load_reals<R, VEC>(sinv + c, sv0);
+      load_reals<R, VEC>(cosv + c + STRIDE, cv1);
+      load_reals<R, VEC>(sinv + c + STRIDE, sv1);
 #pragma unroll
-      for (int q = 0; q < VEC; ++q) {
-        accC[r] = rfma<Acc>(m[q], (Acc)cv0[q], accC[r]);
-        accS[r] = rfma<Acc>(m[q], (Acc)sv0[q], accS[r]);
-      }
-      unpack<S>(raw1[r], m);
+      for (int r = 0; r < ROWS; ++r) {
+        Acc m[VEC];
+        unpack<S>(raw0[r], m);
 #pragma unroll
-      for (int q = 0; q < VEC; ++q) {
-        accC[r] = rfma<Acc>(m[q], (Acc)cv1[q], accC[r]);
-        accS[r] = rfma<Acc>(m[q], (Acc)sv1[q], accS[r]);
+        for (int q = 0; q < VEC; ++q) {
+          accC[r] = rfma<Acc>(m[q], (Acc)cv0[q], accC[r]);
+          accS[r] = rfma<Acc>(m[q], (Acc)sv0[q], accS[r]);
+        }
+        unpack<S>(raw1[r], m);
+#pragma unroll
+        for (int q = 0; q < VEC; ++q) {
+          accC[r] = rfma<Acc>(m[q], (Acc)cv1[q], accC[r]);
+          accS[r] = rfma<Acc>(m[q], (Acc)sv1[q], accS[r]);
+        }
       }
     }
-  }
-  if (c < a.lda) {
-    R cv0[VEC], sv0[VEC];
-    u32x4 raw0[ROWS];
+    if (c < a.lda) {
+      R cv0[VEC], sv0[VEC];
+      u32x4 raw0[ROWS];
 #pragma unroll
-    for (int r = 0; r < ROWS; ++r) raw0[r] = load_row(Ab + r * a.lda + c);
-    load_reals<R, VEC>(cosv + c, cv0);
-    load_reals<R, VEC>(sinv + c, sv0);
+      for (int r = 0; r < ROWS; ++r) raw0[r] = load_row(rowp[r] + c);
+      load_reals<R, VEC>(cosv + c, cv0);
+      load_reals<R, VEC>(sinv + c, sv0);
 #pragma unroll
-    for (int r = 0; r < ROWS; ++r) {
-      Acc m[VEC];
-      unpack<S>(raw0[r], m);
+      for (int r = 0; r < ROWS; ++r) {
+        Acc m[VEC];
+        unpack<S>(raw0[r], m);
 #pragma unroll
-      for (int q = 0; q < VEC; ++q) {
-        accC[r] = rfma<Acc>(m[q], (Acc)cv0[q], accC[r]);
-        accS[r] = rfma<Acc>(m[q], (Acc)sv0[q], accS[r]);
+        for (int q = 0; q < VEC; ++q) {
+          accC[r] = rfma<Acc>(m[q], (Acc)cv0[q], accC[r]);
+          accS[r] = rfma<Acc>(m[q], (Acc)sv0[q], accS[r]);
+        }
       }
     }
+    esum += finish_group<R, Acc, ROWS, KIND>(accC, accS, row0, row0 + nr, a.mscale, a.v);
+    if constexpr (BAL) __syncthreads();  // the next group reuses the reduction buffers
   }
-
-  finish_rows<R, Acc, ROWS, KIND>(accC, accS, row0, a.mscale, a.v);
+  if constexpr (KIND == 7) {
+    if (tid == 0) a.v.err_part[blockIdx.x] = esum;
+  }
 }
 
 // ---------------------------------------------------------------------

@@ -157,3 +157,41 @@ def test_order_parameter(km):
     assert abs(r - exact) < 1e-12
     with km.Integrator(N, km.F64) as it:
         assert abs(it.order(np.full(N, 0.3)) - 1.0) < 1e-14
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("N", [1337, 6000])
+def test_stage_variants_agree(km, orc, prec, N):
+    """All 12 dense stage-kernel variants (rows per group 4/8/16 x
+    non-temporal loads x balanced grid) sum every row in the same order: the
+    RHS and couplings are bit-identical, and whole integrations agree (the
+    balanced grid groups the error partials differently, so the norm may move
+    in its last bit)."""
+    p = km.F32 if prec == "f32" else km.F64
+    dtype = np.float32 if prec == "f32" else np.float64
+    X = km.factor(N, 5, 21, gaussian=True).astype(dtype)
+    w = km.factor_weights(p, N, 5, gaussian=True)
+    th, om = km.draws(p, N, 21, normal=True, init=6.283185307)
+    te = np.arange(1, 5, dtype=np.float64) * 0.25
+    print(f"case variants N={N} prec={prec}", flush=True)
+    with km.Integrator(N, p, atol=1e-6 if prec == "f32" else 1e-9, c1=3.0) as it:
+        it.set_omega(om)
+        it.set_factor(X, w)
+        it.build_adjacency()
+        ref = None
+        runs = {}
+        for rows in (4, 8, 16):
+            for nt in (False, True):
+                for bal in (False, True):
+                    it.select_variant(rows, nt, bal)
+                    assert it.layout()[2:] == (rows, nt, bal)
+                    f, cp = it.rhs(th, couplings=True)
+                    if ref is None:
+                        ref = (f, cp)
+                    assert np.array_equal(f, ref[0]) and np.array_equal(cp, ref[1]), (rows, nt, bal)
+                    if rows == 8 and nt:
+                        it.set_state(th, 0.0, 0.01)
+                        runs[bal] = it.run(te, 1.0, keep_rows=True)
+    a, b = runs[False], runs[True]
+    assert abs(a["status"].accepted - b["status"].accepted) <= 1
+    assert np.max(np.abs(a["rows"][-1] - b["rows"][-1])) < (1e-4 if prec == "f32" else 1e-10)
