@@ -183,7 +183,7 @@ def batch_main(args):
         extra = {"sequential_plain_fp32_ms_per_attempt_x64": 64 * seq_ms,
                  "batched_ms_per_attempt": per_attempt_batched,
                  "speedup_vs_64_sequential_plain": 64 * seq_ms / per_attempt_batched,
-                 "tile": {"n_tiles_per_wave": r["tune"][0], "split_k": r["tune"][1]}}
+                 "tile": {"workgroup_variant": r["tune"][0], "split_k": r["tune"][1]}}
     else:
         alg = sum(2 * n * n + 3 * 4 * n for n in Ns)
         # the learned proposer changes how many attempts reach a given t, so

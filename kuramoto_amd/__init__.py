@@ -104,6 +104,7 @@ _SIGNATURES = {
     "kmt_batch_time_attempts": ([_P, _I, _P, _P, _P], _I),
     "kmt_batch_set_vector_split": ([_P, _I], _I),
     "kmt_batch_tune_table": ([_P, _P], _I),
+    "kmt_batch_select_tile": ([_P, _I, _I], _I),
     "kmt_batch_set_proposer": ([_P, _P, _I, _I], _I),
     "kmt_proposer_param_count": ([], _I),
     "kmt_batch_set_attempt_log": ([_P, _I], _I),
@@ -488,10 +489,14 @@ class Batch:
                "kmt_batch_autotune")
         return nt.value, ks.value, ms.value
 
+    def select_tile(self, variant, split_k):
+        _check(self.lib.kmt_batch_select_tile(self.ptr, variant, split_k), "kmt_batch_select_tile")
+
     def tune_table(self):
-        ms = np.zeros(15)
+        """ms per product, [workgroup shape 0..4][split-K 1, 2, 4, 8, 16]."""
+        ms = np.zeros(25)
         _check(self.lib.kmt_batch_tune_table(self.ptr, _ptr(ms)), "kmt_batch_tune_table")
-        return ms.reshape(3, 5)
+        return ms.reshape(5, 5)
 
     def time_attempts(self, attempts):
         region, coup = ctypes.c_double(), ctypes.c_double()

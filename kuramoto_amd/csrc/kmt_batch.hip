@@ -25,6 +25,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/kuramoto_amd.h"
@@ -133,10 +134,16 @@ template <> __device__ __forceinline__ f32x16_t mfma32<f16_t>(const u32x4 a, con
                                                 0, 0, 0);
 }
 
-template <class T, int MT, int NT, int WAVES, bool LO>
+template <int S> using Stage = std::integral_constant<int, S>;
+
+// PF = chunks of A in flight ahead of the one being multiplied (register
+// sets: PF + 1, used round-robin; the loop is unrolled PF + 1 times so that
+// every set is addressed statically).
+template <class T, int MT, int NT, int WAVES, bool LO, int PF>
 __global__ __launch_bounds__(WAVES * kWave) void bgemm_kernel(const T *At, long nkc, const uint16_t *vthi,
                                                               const uint16_t *vtlo, long ldk, int ksplit,
                                                               long slab_chunks, float *P, long ldn, int mpad) {
+  static_assert(PF >= 1 && PF <= 3, "prefetch depth");
   constexpr int NTHR = WAVES * kWave;
   constexpr int NV = LO ? 2 : 1;
   constexpr int VPIECES = MT * 32 * 8;  // 16-B pieces of one V^T chunk (hi or lo)
@@ -160,7 +167,7 @@ __global__ __launch_bounds__(WAVES * kWave) void bgemm_kernel(const T *At, long 
 #pragma unroll
       for (int v = 0; v < 16; ++v) acc[mt][nt][v] = 0.f;
 
-  u32x4 af[NT][4], an[NT][4];
+  u32x4 a[PF + 1][NT][4];
   u32x4 vr[NV][VPT];
   auto load_a = [&](long c, u32x4 (&dst)[NT][4]) {
 #pragma unroll
@@ -191,24 +198,7 @@ __global__ __launch_bounds__(WAVES * kWave) void bgemm_kernel(const T *At, long 
           *reinterpret_cast<u32x4 *>(&sh[buf][q][(p >> 3) * kLdsRow + (p & 7) * 8]) = vr[q][u];
       }
   };
-
-  if (c0 < c1) {
-    load_a(c0, af);
-    load_v(c0);
-    store_v(0);
-  }
-  // Wait for every prologue load here: the loop below then only waits for
-  // the prefetch it issues itself (without this, the wait inserted for the
-  // prologue's fragments inside the loop also drains the next chunk's loads).
-  __builtin_amdgcn_s_waitcnt(0);
-  __syncthreads();
-  for (long c = c0; c < c1; ++c) {
-    const int cur = (int)((c - c0) & 1);
-    const bool more = c + 1 < c1;
-    if (more) {
-      load_a(c + 1, an);
-      load_v(c + 1);
-    }
+  auto multiply = [&](const u32x4 (&af)[NT][4], int cur) {
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) {
       const int off = (lane % 32) * kLdsRow + 16 * ks + 8 * (lane / 32);
@@ -221,13 +211,49 @@ __global__ __launch_bounds__(WAVES * kWave) void bgemm_kernel(const T *At, long 
           for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = mfma32<T>(vv, af[nt][ks], acc[mt][nt]);
         }
     }
+  };
+  // One chunk c held in register set S (= (c - c0) mod (PF + 1)): the next
+  // V^T chunk's loads are issued first (its wait before the LDS store then
+  // leaves the A loads in flight), then A for chunk c + PF, then the MFMAs.
+  auto step = [&](auto stage, long c) {
+    constexpr int S = decltype(stage)::value;
+    constexpr int NX = (S + PF) % (PF + 1);
+    const int cur = (int)((c - c0) & 1);
+    const bool more = c + 1 < c1;
+    if (more) load_v(c + 1);
+    if (c + PF < c1) load_a(c + PF, a[NX]);
+    multiply(a[S], cur);
     if (more) store_v(cur ^ 1);
     __syncthreads();
-#pragma unroll
-    for (int nt = 0; nt < NT; ++nt)
-#pragma unroll
-      for (int ks = 0; ks < 4; ++ks) af[nt][ks] = an[nt][ks];
+  };
+
+  if (c0 < c1) {
+    load_a(c0, a[0]);
+    if constexpr (PF >= 2)
+      if (c0 + 1 < c1) load_a(c0 + 1, a[1]);
+    if constexpr (PF >= 3)
+      if (c0 + 2 < c1) load_a(c0 + 2, a[2]);
+    load_v(c0);
+    store_v(0);
   }
+  // Wait for every prologue load here: the loop below then only waits for
+  // the prefetch it issues itself (without this, the wait inserted for the
+  // prologue's fragments inside the loop also drains the later chunks' loads).
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  long c = c0;
+  for (; c + PF + 1 <= c1; c += PF + 1) {
+    step(Stage<0>{}, c);
+    step(Stage<1>{}, c + 1);
+    if constexpr (PF >= 2) step(Stage<2>{}, c + 2);
+    if constexpr (PF >= 3) step(Stage<3>{}, c + 3);
+  }
+  if (c < c1) step(Stage<0>{}, c);
+  if constexpr (PF >= 2)
+    if (c + 1 < c1) step(Stage<1>{}, c + 1);
+  if constexpr (PF >= 3)
+    if (c + 2 < c1) step(Stage<2>{}, c + 2);
+
   float *Pk = P + (size_t)slab * mpad * ldn;
 #pragma unroll
   for (int mt = 0; mt < MT; ++mt)
@@ -751,8 +777,8 @@ struct kmt_batch {
   float t1 = 0.f;
   int begun = 0;
   // tiling (shared) / rows (grouped)
-  int tile_rows = 2, ksplit = 8, lo = 1;
-  double tune_ms[16] = {0};  // (tile_rows in 1,2,4) x (split in 1,2,4,8,16)
+  int tile_var = 3, ksplit = 8, lo = 1;
+  double tune_ms[25] = {0};  // (tile_var 0..4) x (split in 1,2,4,8,16)
   hipGraphExec_t g1 = nullptr, g8 = nullptr;
   // proposer
   uint16_t *mlp_w = nullptr;
@@ -965,23 +991,27 @@ struct kmt_batch {
   }
 
   // ---- coupling launches ----
-  // shared product variants: rows of A per workgroup = 128 tile_rows (1: 4
-  // waves x 1 band, 2: 8 waves x 1 band, 4: 8 waves x 2 bands)
-  template <class T, int MT, int NT, int WAVES, bool LO> void gemm_v() {
+  // shared product variants (tile_var): 0 = 4 waves x 1 band (128 rows of A
+  // per workgroup), 1 = 8 waves x 1 band (256), 2 = 8 waves x 2 bands (512),
+  // 3 / 4 = 8 waves x 1 band with two / three chunks of A in flight
+  template <class T, int MT, int NT, int WAVES, bool LO, int PF> void gemm_v() {
     const long nkc = lda_shared / kKC;
     const long slab_chunks = (nkc + ksplit - 1) / ksplit;
     const long nrb = rows_shared / (32L * NT * WAVES);
-    bgemm_kernel<T, MT, NT, WAVES, LO><<<(unsigned)(nrb * ksplit), WAVES * kWave, 0, stream>>>(
+    bgemm_kernel<T, MT, NT, WAVES, LO, PF><<<(unsigned)(nrb * ksplit), WAVES * kWave, 0, stream>>>(
         (const T *)A, nkc, vthi, vtlo, lda_shared, ksplit, slab_chunks, P, ldn, mpad);
   }
-  template <class T, int MT> void gemm_mt() {
-    if (tile_rows == 4) {
-      if (lo) gemm_v<T, MT, 2, 8, true>(); else gemm_v<T, MT, 2, 8, false>();
-    } else if (tile_rows == 2) {
-      if (lo) gemm_v<T, MT, 1, 8, true>(); else gemm_v<T, MT, 1, 8, false>();
-    } else {
-      if (lo) gemm_v<T, MT, 1, 4, true>(); else gemm_v<T, MT, 1, 4, false>();
+  template <class T, int MT, bool LO> void gemm_lo() {
+    switch (tile_var) {
+      case 0: gemm_v<T, MT, 1, 4, LO, 1>(); break;
+      case 1: gemm_v<T, MT, 1, 8, LO, 1>(); break;
+      case 3: gemm_v<T, MT, 1, 8, LO, 2>(); break;
+      case 4: gemm_v<T, MT, 1, 8, LO, 3>(); break;
+      default: gemm_v<T, MT, 2, 8, LO, 1>(); break;
     }
+  }
+  template <class T, int MT> void gemm_mt() {
+    if (lo) gemm_lo<T, MT, true>(); else gemm_lo<T, MT, false>();
   }
   template <class T> void gemm_t() {
     if (mt == 1) gemm_mt<T, 1>();
@@ -1288,10 +1318,13 @@ struct kmt_batch {
     return KMT_OK;
   }
 
-  // Tile autotuner of the shared (MFMA) product: rows per workgroup {128,
-  // 256, 512} x split-K {1, 2, 4, 8, 16}.  The split changes the order in
-  // which partial sums are added (results agree to FP32 rounding); the rows
-  // per workgroup do not change results.
+  // Tile autotuner of the shared (MFMA) product: the five workgroup shapes
+  // (tile_var) x split-K {1, 2, 4, 8, 16}.  The split changes the order in
+  // which partial sums are added (results agree to FP32 rounding); the
+  // workgroup shape does not change results.  Each candidate is scored by
+  // its measured time plus the time to move its split-K partials (written
+  // here, read back by the epilogue: 2 x ksplit x mpad x ldn x 4 bytes at
+  // 6 TB/s), so a wider split has to pay for its extra traffic.
   int autotune(int reps, int *best_nt, int *best_split, double *best_ms) {
     if (!shared) return KMT_OK;
     if (!A) return kmt_fail(KMT_ERR_STATE, "no adjacency");
@@ -1300,12 +1333,12 @@ struct kmt_batch {
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
     double best = 1e30;
-    int bnt = 1, bks = 4;
+    int bnt = 3, bks = 8;
     const int splits[5] = {1, 2, 4, 8, 16};
     int idx = 0;
-    for (int nt = 1; nt <= 4; nt *= 2)
+    for (int nt = 0; nt < 5; ++nt)
       for (int si = 0; si < 5; ++si, ++idx) {
-        tile_rows = nt;
+        tile_var = nt;
         ksplit = splits[si];
         gemm_t_any();
         HIP_TRY(hipEventRecord(e0, stream));
@@ -1315,16 +1348,28 @@ struct kmt_batch {
         float ms = 0;
         HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
         tune_ms[idx] = ms / reps;
-        if (ms < best) { best = ms; bnt = nt; bks = ksplit; }
+        const double partial_ms = 2.0 * ksplit * mpad * (double)ldn * 4.0 / 6e12 * 1e3;
+        const double score = ms / reps + partial_ms;
+        if (score < best) { best = score; bnt = nt; bks = ksplit; }
       }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
-    tile_rows = bnt;
+    tile_var = bnt;
     ksplit = bks;
     drop_graphs();
     if (best_nt) *best_nt = bnt;
     if (best_split) *best_split = bks;
-    if (best_ms) *best_ms = best / reps;
+    if (best_ms) *best_ms = tune_ms[bnt * 5 + (bks == 1 ? 0 : bks == 2 ? 1 : bks == 4 ? 2 : bks == 8 ? 3 : 4)];
+    return KMT_OK;
+  }
+  int select_tile(int variant, int split) {
+    if (!shared) return kmt_fail(KMT_ERR_STATE, "tile selection applies to shared (MFMA) batches");
+    if (variant < 0 || variant > 4) return kmt_fail(KMT_ERR_ARG, "workgroup variant must be 0..4 (got %d)", variant);
+    if (split != 1 && split != 2 && split != 4 && split != 8 && split != 16)
+      return kmt_fail(KMT_ERR_ARG, "split_k must be 1, 2, 4, 8 or 16 (got %d)", split);
+    tile_var = variant;
+    ksplit = split;
+    drop_graphs();
     return KMT_OK;
   }
   void gemm_t_any() {
@@ -1453,9 +1498,13 @@ int kmt_batch_time_attempts(kmt_batch *bp, int attempts, double *region_ms, doub
   if (attempts < 1) return kmt_fail(KMT_ERR_ARG, "attempts < 1");
   return bp->time_attempts(attempts, region_ms, coupling_ms, st);
 }
-int kmt_batch_tune_table(kmt_batch *bp, double *ms15) {
+int kmt_batch_select_tile(kmt_batch *bp, int variant, int split_k) {
+  if (!bp) return kmt_fail(KMT_ERR_ARG, "batch is NULL");
+  return bp->select_tile(variant, split_k);
+}
+int kmt_batch_tune_table(kmt_batch *bp, double *ms25) {
   B_CHECK;
-  if (ms15) for (int q = 0; q < 15; ++q) ms15[q] = bp->tune_ms[q];
+  if (ms25) for (int q = 0; q < 25; ++q) ms25[q] = bp->tune_ms[q];
   return KMT_OK;
 }
 int kmt_batch_set_vector_split(kmt_batch *bp, int hi_lo) {

@@ -116,8 +116,8 @@ def test_shared_autotune_and_split(km):
                 th, om = km.draws(km.F32, N, 30 + b, normal=True, init=6.283185307)
                 bt.set_instance(b, om, th, 3.0, 0.0, 0.01)
             bt.set_factor(-1, X, w)
-            nt, ks, ms = bt.autotune(2)
-            assert nt in (1, 2, 4) and ks in (1, 2, 4, 8, 16) and ms > 0
+            var, ks, ms = bt.autotune(2)
+            assert var in (0, 1, 2, 3, 4) and ks in (1, 2, 4, 8, 16) and ms > 0
             assert np.all(bt.tune_table() > 0)
             bt.set_vector_split(split)
             out[split] = bt.run(te, 1.0)
@@ -153,3 +153,29 @@ def test_attempt_log_and_proposer_fallback(km):
             assert st.proposer_accepts + st.proposer_rejects > 0
     for b in range(B):
         assert np.max(np.abs(runs["standard"]["r"][b] - runs["proposer"]["r"][b])) < 1e-3
+
+
+@pytest.mark.parametrize("storage", ["bf16", "f16"])
+def test_shared_workgroup_variants_agree(km, storage):
+    """The five workgroup shapes of the MFMA product sum every output in the
+    same order (chunk, k-step, hi then lo) within a split-K slab: at a fixed
+    split they give bit-identical runs."""
+    B, N = 5, 1500
+    X = km.factor(N, 5, 9, gaussian=True).astype(np.float32)
+    w = km.factor_weights(km.F32, N, 5, gaussian=True)
+    te = _grid(1.0, 0.25)
+    out = []
+    for var in range(5):
+        print(f"case shared variant {var} {storage}", flush=True)
+        with km.Batch([N] * B, shared=True, storage=storage) as bt:
+            for b in range(B):
+                th, om = km.draws(km.F32, N, 40 + b, normal=True, init=6.283185307)
+                bt.set_instance(b, om, th, 1.0 + b, 0.0, 0.01)
+            bt.set_factor(-1, X, w)
+            bt.select_tile(var, 4)
+            res = bt.run(te, 1.0, chunk=8)
+            out.append((res, [bt.get_instance(b) for b in range(B)]))
+    for var in range(1, 5):
+        for b in range(B):
+            assert np.array_equal(out[var][0]["r"][b], out[0][0]["r"][b]), (var, b)
+            assert np.array_equal(out[var][1][b][0], out[0][1][b][0]), (var, b)
