@@ -301,8 +301,6 @@ template <class R> struct Engine : EngineBase {
   size_t elem_bytes(int stg) const { return (stg == KMT_ADJ_BF16 || stg == KMT_ADJ_F16) ? 2 : sizeof(R); }
 
   int ensure_A(int stg) {
-    if ((stg == KMT_ADJ_BF16 || stg == KMT_ADJ_F16) && sizeof(R) != 4)
-      return fail(KMT_ERR_ARG, "reduced-precision adjacency storage needs the FP32 working precision");
     if (stg != KMT_ADJ_NATIVE && stg != KMT_ADJ_BF16 && stg != KMT_ADJ_F16)
       return fail(KMT_ERR_ARG, "unknown adjacency storage %d", stg);
     const size_t bytes = (size_t)rows_alloc * (size_t)lda * elem_bytes(stg);
@@ -373,12 +371,9 @@ template <class R> struct Engine : EngineBase {
       for (long j = 0; j < N; ++j) mx = std::max(mx, std::fabs((double)M[(size_t)i * ld + j]));
     mscale_ = pow2_at_least(mx);
     factored = 0;
-    if constexpr (sizeof(R) == 4) {
-      const R inv = (R)(1.0 / mscale_);
-      if (stg == KMT_ADJ_BF16) return upload_rows<bf16_t>(M, ld, inv);
-      return upload_rows<f16_t>(M, ld, inv);
-    }
-    return fail(KMT_ERR_ARG, "unreachable storage");
+    const R inv = (R)(1.0 / mscale_);
+    if (stg == KMT_ADJ_BF16) return upload_rows<bf16_t>(M, ld, inv);
+    return upload_rows<f16_t>(M, ld, inv);
   }
 
   int set_factor(const void *Xv, const void *wv, int k_) override {
@@ -441,12 +436,9 @@ template <class R> struct Engine : EngineBase {
       mx = std::max(mx, s);
     }
     mscale_ = pow2_at_least(mx);
-    if constexpr (sizeof(R) == 4) {
-      const R inv = (R)(1.0 / mscale_);
-      if (stg == KMT_ADJ_BF16) return gen_rows<bf16_t>(inv);
-      return gen_rows<f16_t>(inv);
-    }
-    return fail(KMT_ERR_ARG, "unreachable storage");
+    const R inv = (R)(1.0 / mscale_);
+    if (stg == KMT_ADJ_BF16) return gen_rows<bf16_t>(inv);
+    return gen_rows<f16_t>(inv);
   }
 
   int use_factored(int on) override {
@@ -480,12 +472,10 @@ template <class R> struct Engine : EngineBase {
       hipError_t e = hipSuccess;
       if (storage == KMT_ADJ_NATIVE)
         read_rows_kernel<R, R><<<grid, kBlock, 0, stream>>>((const R *)A + (size_t)r0 * lda, lda, N, nr, (R)1, stage);
-      else if constexpr (sizeof(R) == 4) {
-        if (storage == KMT_ADJ_BF16)
-          read_rows_kernel<R, bf16_t><<<grid, kBlock, 0, stream>>>((const bf16_t *)A + (size_t)r0 * lda, lda, N, nr, (R)mscale_, stage);
-        else
-          read_rows_kernel<R, f16_t><<<grid, kBlock, 0, stream>>>((const f16_t *)A + (size_t)r0 * lda, lda, N, nr, (R)mscale_, stage);
-      }
+      else if (storage == KMT_ADJ_BF16)
+        read_rows_kernel<R, bf16_t><<<grid, kBlock, 0, stream>>>((const bf16_t *)A + (size_t)r0 * lda, lda, N, nr, (R)mscale_, stage);
+      else
+        read_rows_kernel<R, f16_t><<<grid, kBlock, 0, stream>>>((const f16_t *)A + (size_t)r0 * lda, lda, N, nr, (R)mscale_, stage);
       e = hipGetLastError();
       if (e == hipSuccess) e = hipMemcpyAsync((R *)out + (size_t)(r0 - i0) * N, stage, sizeof(R) * nr * N, hipMemcpyDeviceToHost, stream);
       if (e == hipSuccess) e = hipStreamSynchronize(stream);
@@ -558,10 +548,8 @@ template <class R> struct Engine : EngineBase {
 
   template <int KIND> void launch_dense(const Vecs<R> &v) {
     if (storage == KMT_ADJ_NATIVE) launch_dense_kind<R, KIND>(v);
-    else if constexpr (sizeof(R) == 4) {
-      if (storage == KMT_ADJ_BF16) launch_dense_kind<bf16_t, KIND>(v);
-      else launch_dense_kind<f16_t, KIND>(v);
-    }
+    else if (storage == KMT_ADJ_BF16) launch_dense_kind<bf16_t, KIND>(v);
+    else launch_dense_kind<f16_t, KIND>(v);
   }
 
   bool lr_fused() const { return K <= kLrMaxRank; }

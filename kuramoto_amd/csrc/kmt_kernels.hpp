@@ -954,6 +954,16 @@ template <> __device__ __forceinline__ f16_t store_value<float, f16_t>(float v, 
   return f16_t{__builtin_bit_cast(uint16_t, (_Float16)(v * s))};
 }
 
+// FP64 working precision with 16-bit storage: the scaled value is rounded
+// to FP32 first, then to 16 bits (two roundings; the stored matrix is what
+// the integration uses and what kmt_get_adjacency_rows returns).
+template <> __device__ __forceinline__ bf16_t store_value<double, bf16_t>(double v, double s) {
+  return bf16_t{to_bf16_rne((float)(v * s))};
+}
+template <> __device__ __forceinline__ f16_t store_value<double, f16_t>(double v, double s) {
+  return f16_t{__builtin_bit_cast(uint16_t, (_Float16)(float)(v * s))};
+}
+
 template <class R, class S> __device__ __forceinline__ R load_value(S v, R scale);
 template <> __device__ __forceinline__ float load_value<float, float>(float v, float) { return v; }
 template <> __device__ __forceinline__ double load_value<double, double>(double v, double) { return v; }
@@ -962,6 +972,12 @@ template <> __device__ __forceinline__ float load_value<float, bf16_t>(bf16_t v,
 }
 template <> __device__ __forceinline__ float load_value<float, f16_t>(f16_t v, float s) {
   return (float)__builtin_bit_cast(_Float16, v.bits) * s;
+}
+template <> __device__ __forceinline__ double load_value<double, bf16_t>(bf16_t v, double s) {
+  return (double)__builtin_bit_cast(float, (uint32_t)v.bits << 16) * s;
+}
+template <> __device__ __forceinline__ double load_value<double, f16_t>(f16_t v, double s) {
+  return (double)(float)__builtin_bit_cast(_Float16, v.bits) * s;
 }
 
 // getadj (kuramoto.cu:145-161) from the stored factor: one workgroup per

@@ -21,7 +21,9 @@
 //                       standard attractive Kuramoto coupling with a c/N
 //                       prefactor) instead of the volcano matrix
 //   --adj-dtype=T       HBM storage of the dense adjacency: native (default),
-//                       bf16 or f16 (FP32 program only; FP32 accumulation)
+//                       bf16 or f16 (FP32 accumulation of the products;
+//                       kuramoto_64 keeps the state, stage combinations and
+//                       error norm in FP64)
 //   --rebuild           with -A: rebuild the adjacency tiles from the factor in
 //                       registers at every stage (dense semantics, O(N^2 K)
 //                       work, no N^2 storage) instead of the O(N K) factored

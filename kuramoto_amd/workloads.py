@@ -17,6 +17,10 @@ PLAIN = {
     # FP64 plain path (the kuramoto_64 arithmetic), the memory-bound gemv case
     "fp64-n50000": dict(N=50000, K=5, seed=2, prec="f64", normal=True, init=6.283185307, c1=3.0, atol=1e-9,
                         t1=100.0, dt=1.0, storage="native"),
+    # the same FP64 run with the adjacency stored in BF16 (5 GB): FP32
+    # accumulation of the products, FP64 state, stage combinations and norm
+    "fp64-n50000-bf16": dict(N=50000, K=5, seed=2, prec="f64", normal=True, init=6.283185307, c1=3.0, atol=1e-9,
+                             t1=100.0, dt=1.0, storage="bf16"),
     # FP32 dense at the same size (10 GB adjacency)
     "fp32-n50000": dict(N=50000, K=5, seed=2, prec="f32", normal=True, init=6.283185307, c1=3.0, atol=1e-6,
                         t1=100.0, dt=1.0, storage="native"),

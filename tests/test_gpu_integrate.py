@@ -286,3 +286,28 @@ def test_rebuilt_tiles_equal_dense(km, prec, K):
     assert rd["status"].accepted == rr["status"].accepted and rd["status"].rejected == rr["status"].rejected
     assert np.array_equal(rd["rows"], rr["rows"]) and np.array_equal(rd["r"], rr["r"])
     assert np.array_equal(sd[0], sr[0]) and sd[1] == sr[1]
+
+
+@pytest.mark.parametrize("storage", ["bf16", "f16"])
+def test_fp64_state_with_16bit_adjacency(km, orc, storage):
+    """The 64-bit program with a BF16/FP16 adjacency: products accumulated in
+    FP32, state update, stage combinations and error norm in FP64.  Against
+    the FP64 oracle on the same (dequantized) matrix: accepted steps within
+    2% (or 1), r(t) within 1e-6 and phases at t=1 within 1e-6 relative (the
+    FP32 accumulation of the coupling is the only reduced-precision step)."""
+    N, K, seed, c1, t1, atol = 1500, 5, 7, 3.0, 1.0, 1e-9
+    dtype, theta0, omega, X, w = _inputs(km, km.F64, N, K, seed, normal=True, init=6.283185307)
+    te = _grid(0.0, t1, 0.1, dtype)
+    print(f"case fp64 + {storage}", flush=True)
+    with km.Integrator(N, km.F64, atol=atol, c1=c1) as it:
+        it.set_omega(omega)
+        it.set_factor(X, w)
+        it.build_adjacency(storage)
+        Mq = it.adjacency_rows(0, N)
+        it.set_state(theta0, 0.0, 0.01)
+        g = it.run(te, t1, keep_rows=True)
+    ref = orc.integrate(orc.Adj(0, M=Mq), omega, c1, theta0, 0.0, 0.01, t1, atol, t_eval=te)
+    assert abs(g["status"].accepted - ref["accepted"]) <= max(1, int(0.02 * ref["accepted"]))
+    assert np.max(np.abs(g["r"] - ref["r"])) < 1e-6
+    rel = np.abs(g["rows"][-1] - ref["rows"][-1]) / np.maximum(1.0, np.abs(ref["rows"][-1]))
+    assert rel.max() < 1e-6, rel.max()

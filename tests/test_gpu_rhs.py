@@ -98,25 +98,31 @@ def test_build_adjacency_bit_exact(km, orc, gauss, prec):
 
 
 @pytest.mark.parametrize("storage", ["bf16", "f16"])
-def test_reduced_storage_rhs(km, orc, storage):
-    """BF16/FP16 storage with FP32 accumulation equals the FP32 kernel applied
-    to the dequantized matrix (checked against the oracle on that matrix)."""
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_reduced_storage_rhs(km, orc, storage, prec):
+    """BF16/FP16 storage with FP32 accumulation of the products equals the
+    working-precision kernel applied to the dequantized matrix up to FP32
+    accumulation (checked against the oracle on that matrix); the FP64
+    program keeps the RHS epilogue in FP64."""
     N, K, seed, c1 = 2049, 5, 3, 3.0
-    X = km.factor(N, K, seed, gaussian=True).astype(np.float32)
-    w = km.factor_weights(km.F32, N, K, gaussian=True)
+    p = km.F32 if prec == "f32" else km.F64
+    dtype = np.float32 if prec == "f32" else np.float64
+    X = km.factor(N, K, seed, gaussian=True).astype(dtype)
+    w = km.factor_weights(p, N, K, gaussian=True)
     rng = np.random.default_rng(5)
-    omega = rng.standard_normal(N).astype(np.float32)
-    theta = rng.uniform(-np.pi, np.pi, N).astype(np.float32)
-    print(f"case reduced storage={storage}", flush=True)
-    with km.Integrator(N, km.F32, c1=c1) as it:
+    omega = rng.standard_normal(N).astype(dtype)
+    theta = rng.uniform(-np.pi, np.pi, N).astype(dtype)
+    print(f"case reduced storage={storage} prec={prec}", flush=True)
+    with km.Integrator(N, p, c1=c1) as it:
         it.set_omega(omega)
         it.set_factor(X, w)
         it.build_adjacency(storage)
         Mq = it.adjacency_rows(0, N)
         scale = it.adjacency_scale()
         f = it.rhs(theta)
+    assert Mq.dtype == dtype
     assert scale > 0 and np.log2(scale) == np.round(np.log2(scale))
-    fo, _ = orc.rhs(orc.Adj(0, M=Mq), omega, c1, theta, dtype=np.float32)
+    fo, _ = orc.rhs(orc.Adj(0, M=Mq.astype(np.float64)), omega.astype(np.float64), c1, theta.astype(np.float64))
     tol = 2e-6 * _scale(Mq, omega, c1)
     assert np.all(np.abs(f.astype(np.float64) - fo) <= tol)
     # quantization error against the exact matrix stays at the format's epsilon
