@@ -69,6 +69,12 @@ const char *kmt_last_error(void);
 /* Device query, used for the reference's memory check (kuramoto.cu:424-438). */
 int kmt_device_count(int *count);
 int kmt_mem_info(int device, size_t *free_bytes, size_t *total_bytes);
+/* Page-locked host buffers for output staging (device-to-host copies into
+ * them run at full PCIe rate and may overlap host work).  No reference
+ * counterpart (the reference copies into pageable vectors, kuramoto.cu:
+ * 221-231). */
+int kmt_host_alloc(size_t bytes, void **ptr);
+int kmt_host_free(void *ptr);
 
 /* ---- host-side generators (no GPU work) ------------------------------- */
 /* main()'s draw order: srand(seed); N initial-condition draws

@@ -64,6 +64,8 @@ _SIGNATURES = {
     "kmt_last_error": ([], ctypes.c_char_p),
     "kmt_device_count": ([_P], _I),
     "kmt_mem_info": ([_I, _P, _P], _I),
+    "kmt_host_alloc": ([ctypes.c_size_t, _P], _I),
+    "kmt_host_free": ([_P], _I),
     "kmt_draws": ([_I, _L, _I, _I, _D, _D, _P, _P], _I),
     "kmt_factor": ([_L, _I, ctypes.c_uint32, _I, _P], _I),
     "kmt_factor_weights": ([_I, _L, _I, _I, _P], _I),

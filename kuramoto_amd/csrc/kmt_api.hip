@@ -1054,6 +1054,20 @@ int kmt_device_count(int *count) {
   return KMT_OK;
 }
 
+int kmt_host_alloc(size_t bytes, void **ptr) {
+  if (!ptr) return fail(KMT_ERR_ARG, "ptr is NULL");
+  *ptr = nullptr;
+  hipError_t e = hipHostMalloc(ptr, bytes > 0 ? bytes : 1, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    *ptr = nullptr;
+    return fail(KMT_ERR_MEMORY, "cannot allocate %zu bytes of pinned host memory: %s", bytes, hipGetErrorString(e));
+  }
+  return KMT_OK;
+}
+int kmt_host_free(void *ptr) {
+  if (ptr) HIP_TRY(hipHostFree(ptr));
+  return KMT_OK;
+}
 int kmt_mem_info(int device, size_t *free_bytes, size_t *total_bytes) {
   HIP_TRY(hipSetDevice(device));
   size_t f = 0, t = 0;

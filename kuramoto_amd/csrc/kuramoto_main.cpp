@@ -37,10 +37,14 @@
 
 #include <algorithm>
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "kuramoto_amd.h"
@@ -118,6 +122,100 @@ static void write_fs(const std::vector<R> &th, R t, R h, const std::vector<R> &f
   std::fwrite(floc.data(), sizeof(R), floc.size(), f);
   std::fclose(f);
 }
+
+// Output files are appended by a writer thread, so the file I/O of one
+// drained chunk overlaps the device's next chunk.  Batches are written in
+// the order they were drained (the same bytes and order as appending in the
+// loop).  Phase rows are staged in page-locked buffers (two, reused) that
+// kmt_read_outputs copies into at full PCIe rate.
+struct OutBatch {
+  std::vector<R> times;        // times.dat
+  R *rows = nullptr;           // thetas.dat rows (a pooled pinned buffer), nrows x n_osc
+  size_t nrows = 0, n_osc = 0;
+  std::vector<R> cpl;          // couplings.dat rows
+  std::vector<R> r;            // order.dat
+  bool fs = false;             // rewrite fs.dat with the state below
+  std::vector<R> fs_th, fs_f;
+  R fs_t = 0, fs_h = 0;
+};
+
+class Writer {
+ public:
+  Writer(size_t row_bytes, int dense) : dense_(dense) {
+    for (int q = 0; q < 2 && row_bytes > 0; ++q) {
+      void *p = nullptr;
+      if (kmt_host_alloc(row_bytes, &p) != KMT_OK) {
+        std::fprintf(stderr, "kuramoto: %s\n", kmt_last_error());
+        std::exit(1);
+      }
+      free_.push_back((R *)p);
+      all_.push_back((R *)p);
+    }
+    th_ = std::thread([this] { run(); });
+  }
+  ~Writer() {
+    finish();
+    for (R *p : all_) kmt_host_free(p);
+  }
+  // a free pinned row buffer (waits for the writer to release one)
+  R *take_rows() {
+    std::unique_lock<std::mutex> lk(mu_);
+    cv_.wait(lk, [this] { return !free_.empty(); });
+    R *p = free_.front();
+    free_.pop_front();
+    return p;
+  }
+  void give_back(R *p) {
+    std::lock_guard<std::mutex> lk(mu_);
+    free_.push_back(p);
+    cv_.notify_all();
+  }
+  void push(OutBatch &&b) {
+    std::lock_guard<std::mutex> lk(mu_);
+    q_.push_back(std::move(b));
+    cv_.notify_all();
+  }
+  // write everything queued, stop the thread
+  void finish() {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      if (done_) return;
+      done_ = true;
+      cv_.notify_all();
+    }
+    th_.join();
+  }
+
+ private:
+  void run() {
+    for (;;) {
+      OutBatch b;
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [this] { return done_ || !q_.empty(); });
+        if (q_.empty()) return;
+        b = std::move(q_.front());
+        q_.pop_front();
+      }
+      if (dense_ >= 1) append("times.dat", b.times.data(), sizeof(R), b.times.size());
+      if (b.nrows > 0) {
+        if (dense_ >= 1) append("thetas.dat", b.rows, sizeof(R), b.nrows * b.n_osc);
+        if (dense_ >= 2) append("couplings.dat", b.cpl.data(), sizeof(R), b.cpl.size());
+        append("order.dat", b.r.data(), sizeof(R), b.r.size());
+      }
+      if (b.rows) give_back(b.rows);
+      if (b.fs) write_fs(b.fs_th, b.fs_t, b.fs_h, b.fs_f);
+    }
+  }
+  int dense_;
+  std::thread th_;
+  std::mutex mu_;
+  std::condition_variable cv_;
+  std::deque<OutBatch> q_;
+  std::deque<R *> free_;
+  std::vector<R *> all_;
+  bool done_ = false;
+};
 
 int main(int argc, char *argv[]) {
   const double start = now_s();
@@ -358,7 +456,8 @@ int main(int argc, char *argv[]) {
   const double loop0 = now_s();
   kmt_status st;
   std::memset(&st, 0, sizeof(st));
-  std::vector<R> ts(4096), hs(4096), rows(keep_rows ? (size_t)ring_rows * N : 0), rv(ring_rows), cpl(2 * N);
+  std::vector<R> ts(4096), hs(4096), rv(ring_rows), cpl(2 * N);
+  Writer writer(keep_rows ? sizeof(R) * (size_t)ring_rows * N : 0, dense);
   long steps = 0;
   int chunk = 1;
   bool started = false, stopped = false;
@@ -368,6 +467,7 @@ int main(int argc, char *argv[]) {
     CALL(kmt_advance(g_ctx, n, &st));
     started = true;
     chunk = std::min(chunk * 2, chunk_max);
+    OutBatch b;
     // per accepted step: verbose line and times.dat (kuramoto.cu:169-188)
     int nt = 0;
     CALL(kmt_read_times(g_ctx, (int)ts.size(), ts.data(), hs.data(), &nt));
@@ -384,35 +484,40 @@ int main(int argc, char *argv[]) {
         std::fclose(fo);
       }
     }
-    if (dense >= 1) append("times.dat", ts.data(), sizeof(R), nt);
+    b.times.assign(ts.begin(), ts.begin() + nt);
     // dense-output rows and order parameters (kuramoto.cu:212-249)
     int nr = 0;
-    CALL(kmt_read_outputs(g_ctx, ring_rows, keep_rows ? rows.data() : nullptr, rv.data(), &nr));
+    if (keep_rows) b.rows = writer.take_rows();
+    CALL(kmt_read_outputs(g_ctx, ring_rows, b.rows, rv.data(), &nr));
+    b.nrows = (size_t)nr;
+    b.n_osc = N;
     if (nr > 0) {
-      if (dense >= 1) append("thetas.dat", rows.data(), sizeof(R), (size_t)nr * N);
       if (dense >= 2) {
-        FILE *fc = std::fopen(fname("couplings.dat").c_str(), "ab");
-        for (int q = 0; q < nr; ++q) {
-          CALL(kmt_rhs(g_ctx, rows.data() + (size_t)q * N, nullptr, cpl.data()));
-          std::fwrite(cpl.data(), sizeof(R), 2 * N, fc);
-        }
-        std::fclose(fc);
-        floc = cpl;
+        b.cpl.resize((size_t)nr * 2 * N);
+        for (int q = 0; q < nr; ++q)
+          CALL(kmt_rhs(g_ctx, b.rows + (size_t)q * N, nullptr, b.cpl.data() + (size_t)q * 2 * N));
+        floc.assign(b.cpl.end() - 2 * N, b.cpl.end());
       }
-      append("order.dat", rv.data(), sizeof(R), nr);
+      b.r.assign(rv.begin(), rv.begin() + nr);
     }
     // restart file (kuramoto.cu:251-264; kuramoto_64.cu writes it always)
     if (nt > 0 && (sizeof(R) == 8 || dense >= 0)) {
       double tt = 0, hh = 0;
       CALL(kmt_get_state(g_ctx, yloc.data(), &tt, &hh));
-      write_fs(yloc, (R)tt, (R)hh, floc);
+      b.fs = true;
+      b.fs_t = (R)tt;
+      b.fs_h = (R)hh;
+      b.fs_f = floc;
+      b.fs_th = yloc;
     }
+    writer.push(std::move(b));
     if (!st.active) break;
     if (max_steps > 0 && steps >= max_steps && st.pending == 0) {
       stopped = true;
       break;
     }
   }
+  writer.finish();
   const double loop_s = now_s() - loop0;
   if (stopped) {  // leave the files as a process killed after this step would
     kmt_destroy(g_ctx);
