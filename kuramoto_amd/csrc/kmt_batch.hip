@@ -778,7 +778,7 @@ struct kmt_batch {
   int begun = 0;
   // tiling (shared) / rows (grouped)
   int tile_var = 3, ksplit = 8, lo = 1;
-  double tune_ms[25] = {0};  // (tile_var 0..4) x (split in 1,2,4,8,16)
+  double tune_ms[30] = {0};  // (tile_var 0..5) x (split in 1,2,4,8,16)
   hipGraphExec_t g1 = nullptr, g8 = nullptr;
   // proposer
   uint16_t *mlp_w = nullptr;
@@ -993,7 +993,8 @@ struct kmt_batch {
   // ---- coupling launches ----
   // shared product variants (tile_var): 0 = 4 waves x 1 band (128 rows of A
   // per workgroup), 1 = 8 waves x 1 band (256), 2 = 8 waves x 2 bands (512),
-  // 3 / 4 = 8 waves x 1 band with two / three chunks of A in flight
+  // 3 / 4 = 8 waves x 1 band with two / three chunks of A in flight,
+  // 5 = 4 waves x 1 band with two chunks in flight
   template <class T, int MT, int NT, int WAVES, bool LO, int PF> void gemm_v() {
     const long nkc = lda_shared / kKC;
     const long slab_chunks = (nkc + ksplit - 1) / ksplit;
@@ -1007,6 +1008,7 @@ struct kmt_batch {
       case 1: gemm_v<T, MT, 1, 8, LO, 1>(); break;
       case 3: gemm_v<T, MT, 1, 8, LO, 2>(); break;
       case 4: gemm_v<T, MT, 1, 8, LO, 3>(); break;
+      case 5: gemm_v<T, MT, 1, 4, LO, 2>(); break;
       default: gemm_v<T, MT, 2, 8, LO, 1>(); break;
     }
   }
@@ -1336,7 +1338,7 @@ struct kmt_batch {
     int bnt = 3, bks = 8;
     const int splits[5] = {1, 2, 4, 8, 16};
     int idx = 0;
-    for (int nt = 0; nt < 5; ++nt)
+    for (int nt = 0; nt < 6; ++nt)
       for (int si = 0; si < 5; ++si, ++idx) {
         tile_var = nt;
         ksplit = splits[si];
@@ -1364,7 +1366,7 @@ struct kmt_batch {
   }
   int select_tile(int variant, int split) {
     if (!shared) return kmt_fail(KMT_ERR_STATE, "tile selection applies to shared (MFMA) batches");
-    if (variant < 0 || variant > 4) return kmt_fail(KMT_ERR_ARG, "workgroup variant must be 0..4 (got %d)", variant);
+    if (variant < 0 || variant > 5) return kmt_fail(KMT_ERR_ARG, "workgroup variant must be 0..5 (got %d)", variant);
     if (split != 1 && split != 2 && split != 4 && split != 8 && split != 16)
       return kmt_fail(KMT_ERR_ARG, "split_k must be 1, 2, 4, 8 or 16 (got %d)", split);
     tile_var = variant;
@@ -1502,9 +1504,9 @@ int kmt_batch_select_tile(kmt_batch *bp, int variant, int split_k) {
   if (!bp) return kmt_fail(KMT_ERR_ARG, "batch is NULL");
   return bp->select_tile(variant, split_k);
 }
-int kmt_batch_tune_table(kmt_batch *bp, double *ms25) {
+int kmt_batch_tune_table(kmt_batch *bp, double *ms30) {
   B_CHECK;
-  if (ms25) for (int q = 0; q < 25; ++q) ms25[q] = bp->tune_ms[q];
+  if (ms30) for (int q = 0; q < 30; ++q) ms30[q] = bp->tune_ms[q];
   return KMT_OK;
 }
 int kmt_batch_set_vector_split(kmt_batch *bp, int hi_lo) {
