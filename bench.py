@@ -180,7 +180,9 @@ def batch_main(args):
         seq_ms = plain_attempt_ms(N)
         inst_steps = st.accepted - st0.accepted
         per_attempt_batched = r["region_ms"] / args.steps
-        extra = {"sequential_plain_fp32_ms_per_attempt_x64": 64 * seq_ms,
+        extra = {"rhs_evals_in_region": st.rhs_evals - st0.rhs_evals,
+                 "rejected_in_region": st.rejected - st0.rejected,
+                 "sequential_plain_fp32_ms_per_attempt_x64": 64 * seq_ms,
                  "batched_ms_per_attempt": per_attempt_batched,
                  "speedup_vs_64_sequential_plain": 64 * seq_ms / per_attempt_batched,
                  "tile": {"workgroup_variant": r["tune"][0], "split_k": r["tune"][1]}}
@@ -281,6 +283,7 @@ def main():
                                + f", atol={cfg['atol']}, c={cfg['c1']}",
                    "N": N, "accepted_in_region": accepted,
                    "rejected_in_region": r["st"].rejected - r["st0"].rejected,
+                   "rhs_evals_in_region": r["st"].rhs_evals - r["st0"].rhs_evals,
                    "t_after": r["st"].t, "adjacency_build_s": round(r["gen_s"], 3)},
         "roofline": {**roof,
                      "traffic": profiled_traffic(args.config, "lowrank_stage_kernel" if cfg["storage"] == "factored"
