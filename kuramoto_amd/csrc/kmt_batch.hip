@@ -198,18 +198,54 @@ __global__ __launch_bounds__(WAVES * kWave) void bgemm_kernel(const T *At, long 
           *reinterpret_cast<u32x4 *>(&sh[buf][q][(p >> 3) * kLdsRow + (p & 7) * 8]) = vr[q][u];
       }
   };
+  // The MFMAs of one chunk: 4 k-steps x (hi, lo) x MT tiles of V^T from
+  // LDS.  With one band per wave (NT = 1) the V^T fragments of k-step ks + 1
+  // are read from LDS while the MFMAs of k-step ks run (two fragment sets),
+  // so the LDS latency is not exposed once per MFMA pair.
   auto multiply = [&](const u32x4 (&af)[NT][4], int cur) {
+    if constexpr (NT == 1) {
+      u32x4 vb[2][NV][MT];
+      auto frag = [&](int ks, u32x4 (&dst)[NV][MT]) {
+        const int off = (lane % 32) * kLdsRow + 16 * ks + 8 * (lane / 32);
 #pragma unroll
-    for (int ks = 0; ks < 4; ++ks) {
-      const int off = (lane % 32) * kLdsRow + 16 * ks + 8 * (lane / 32);
+        for (int q = 0; q < NV; ++q)
 #pragma unroll
-      for (int q = 0; q < NV; ++q)
+          for (int mt = 0; mt < MT; ++mt)
+            dst[q][mt] = *reinterpret_cast<const u32x4 *>(&sh[cur][q][mt * 32 * kLdsRow + off]);
+      };
+      frag(0, vb[0]);
 #pragma unroll
-        for (int mt = 0; mt < MT; ++mt) {
-          const u32x4 vv = *reinterpret_cast<const u32x4 *>(&sh[cur][q][mt * 32 * kLdsRow + off]);
+      for (int ks = 0; ks < 4; ++ks) {
+        if (ks + 1 < 4) frag(ks + 1, vb[(ks + 1) & 1]);
 #pragma unroll
-          for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = mfma32<T>(vv, af[nt][ks], acc[mt][nt]);
-        }
+        for (int q = 0; q < NV; ++q)
+#pragma unroll
+          for (int mt = 0; mt < MT; ++mt) acc[mt][0] = mfma32<T>(vb[ks & 1][q][mt], af[0][ks], acc[mt][0]);
+      }
+      // scheduling: the first k-step's reads, then one read of the next
+      // k-step per MFMA of the current one (mask 0x100 = LDS read, 0x008 =
+      // MFMA), then the last k-step's MFMAs
+      constexpr int F = NV * MT;
+      __builtin_amdgcn_sched_group_barrier(0x100, F, 0);
+#pragma unroll
+      for (int i = 0; i < 3 * F; ++i) {
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+      }
+      __builtin_amdgcn_sched_group_barrier(0x008, F, 0);
+    } else {
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks) {
+        const int off = (lane % 32) * kLdsRow + 16 * ks + 8 * (lane / 32);
+#pragma unroll
+        for (int q = 0; q < NV; ++q)
+#pragma unroll
+          for (int mt = 0; mt < MT; ++mt) {
+            const u32x4 vv = *reinterpret_cast<const u32x4 *>(&sh[cur][q][mt * 32 * kLdsRow + off]);
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = mfma32<T>(vv, af[nt][ks], acc[mt][nt]);
+          }
+      }
     }
   };
   // One chunk c held in register set S (= (c - c0) mod (PF + 1)): the next
