@@ -247,12 +247,12 @@ int kmt_batch_get_instance(kmt_batch *batch, int b, float *theta, double *state8
 /* tile autotuner of the shared MFMA product: workgroup shape `variant`
  * (0: 4 waves x 32 rows of A, 1: 8 waves x 32 rows, 2: 8 waves x 64 rows,
  * 3 / 4: 8 waves x 32 rows with two / three 64-wide chunks of A in flight,
- * 5: 4 waves x 32 rows with two chunks in flight) x split-K {1, 2, 4, 8,
- * 16}, `reps` launches each; keeps the fastest (time plus the split-K
- * partial traffic).  kmt_batch_tune_table returns the 30 average times
- * (ms), variant major. */
+ * 5: 4 waves x 32 rows with two chunks in flight) x split-K 1..16, `reps`
+ * launches each; keeps the fastest (time plus the split-K partial
+ * traffic).  kmt_batch_tune_table returns the 96 average times (ms),
+ * variant major: ms96[16 variant + split_k - 1]. */
 int kmt_batch_autotune(kmt_batch *batch, int reps, int *variant, int *split_k, double *best_ms);
-int kmt_batch_tune_table(kmt_batch *batch, double *ms30);
+int kmt_batch_tune_table(kmt_batch *batch, double *ms96);
 /* pin the shared product's workgroup variant and split-K instead of the
  * autotuner's choice */
 int kmt_batch_select_tile(kmt_batch *batch, int variant, int split_k);

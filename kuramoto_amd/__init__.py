@@ -495,10 +495,10 @@ class Batch:
         _check(self.lib.kmt_batch_select_tile(self.ptr, variant, split_k), "kmt_batch_select_tile")
 
     def tune_table(self):
-        """ms per product, [workgroup shape 0..5][split-K 1, 2, 4, 8, 16]."""
-        ms = np.zeros(30)
+        """ms per product, [workgroup shape 0..5][split-K 1..16]."""
+        ms = np.zeros(96)
         _check(self.lib.kmt_batch_tune_table(self.ptr, _ptr(ms)), "kmt_batch_tune_table")
-        return ms.reshape(6, 5)
+        return ms.reshape(6, 16)
 
     def time_attempts(self, attempts):
         region, coup = ctypes.c_double(), ctypes.c_double()

@@ -139,11 +139,18 @@ template <int S> using Stage = std::integral_constant<int, S>;
 // PF = chunks of A in flight ahead of the one being multiplied (register
 // sets: PF + 1, used round-robin; the loop is unrolled PF + 1 times so that
 // every set is addressed statically).
-template <class T, int MT, int NT, int WAVES, bool LO, int PF>
+template <int I, int U, class F> __device__ __forceinline__ void static_for(F &&f) {
+  if constexpr (I < U) {
+    f(Stage<I>{});
+    static_for<I + 1, U>(f);
+  }
+}
+
+template <class T, int MT, int NT, int WAVES, bool LO, int PF, int VD>
 __global__ __launch_bounds__(WAVES * kWave) void bgemm_kernel(const T *At, long nkc, const uint16_t *vthi,
                                                               const uint16_t *vtlo, long ldk, int ksplit,
                                                               long slab_chunks, float *P, long ldn, int mpad) {
-  static_assert(PF >= 1 && PF <= 3, "prefetch depth");
+  static_assert(PF >= 1 && PF <= 3 && (VD == 1 || VD == 2), "prefetch depth");
   constexpr int NTHR = WAVES * kWave;
   constexpr int NV = LO ? 2 : 1;
   constexpr int VPIECES = MT * 32 * 8;  // 16-B pieces of one V^T chunk (hi or lo)
@@ -168,7 +175,7 @@ __global__ __launch_bounds__(WAVES * kWave) void bgemm_kernel(const T *At, long 
       for (int v = 0; v < 16; ++v) acc[mt][nt][v] = 0.f;
 
   u32x4 a[PF + 1][NT][4];
-  u32x4 vr[NV][VPT];
+  u32x4 vr[VD][NV][VPT];
   auto load_a = [&](long c, u32x4 (&dst)[NT][4]) {
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) {
@@ -178,7 +185,7 @@ __global__ __launch_bounds__(WAVES * kWave) void bgemm_kernel(const T *At, long 
         dst[nt][ks] = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(tile + ks * 512));
     }
   };
-  auto load_v = [&](long c) {
+  auto load_v = [&](long c, u32x4 (&vr)[NV][VPT]) {
 #pragma unroll
     for (int q = 0; q < NV; ++q)
 #pragma unroll
@@ -188,7 +195,7 @@ __global__ __launch_bounds__(WAVES * kWave) void bgemm_kernel(const T *At, long 
           vr[q][u] = *reinterpret_cast<const u32x4 *>(vsrc[q] + (size_t)(p >> 3) * ldk + c * kKC + (p & 7) * 8);
       }
   };
-  auto store_v = [&](int buf) {
+  auto store_v = [&](const u32x4 (&vr)[NV][VPT], int buf) {
 #pragma unroll
     for (int q = 0; q < NV; ++q)
 #pragma unroll
@@ -248,18 +255,24 @@ __global__ __launch_bounds__(WAVES * kWave) void bgemm_kernel(const T *At, long 
       }
     }
   };
-  // One chunk c held in register set S (= (c - c0) mod (PF + 1)): the next
-  // V^T chunk's loads are issued first (its wait before the LDS store then
-  // leaves the A loads in flight), then A for chunk c + PF, then the MFMAs.
+  // One chunk c held in A register set S (= (c - c0) mod (PF + 1)).  V^T is
+  // loaded VD chunks ahead into register set (c - c0) mod VD and written to
+  // LDS one chunk ahead: the loads of chunk c + VD are issued first, then A
+  // for chunk c + PF, then the MFMAs of chunk c, then the LDS store of chunk
+  // c + 1 (whose wait leaves every later load in flight).  The loop is
+  // unrolled over a common period of both register rotations so that every
+  // set is addressed statically.
+  constexpr int U = (PF + 1) % VD == 0 ? PF + 1 : VD * (PF + 1);
   auto step = [&](auto stage, long c) {
-    constexpr int S = decltype(stage)::value;
+    constexpr int I = decltype(stage)::value;
+    constexpr int S = I % (PF + 1);
     constexpr int NX = (S + PF) % (PF + 1);
+    constexpr int SV = I % VD;
     const int cur = (int)((c - c0) & 1);
-    const bool more = c + 1 < c1;
-    if (more) load_v(c + 1);
+    if (c + VD < c1) load_v(c + VD, vr[SV]);
     if (c + PF < c1) load_a(c + PF, a[NX]);
     multiply(a[S], cur);
-    if (more) store_v(cur ^ 1);
+    if (c + 1 < c1) store_v(vr[(SV + 1) % VD], cur ^ 1);
     __syncthreads();
   };
 
@@ -269,8 +282,10 @@ __global__ __launch_bounds__(WAVES * kWave) void bgemm_kernel(const T *At, long 
       if (c0 + 1 < c1) load_a(c0 + 1, a[1]);
     if constexpr (PF >= 3)
       if (c0 + 2 < c1) load_a(c0 + 2, a[2]);
-    load_v(c0);
-    store_v(0);
+    load_v(c0, vr[0]);
+    store_v(vr[0], 0);
+    if constexpr (VD == 2)
+      if (c0 + 1 < c1) load_v(c0 + 1, vr[1]);
   }
   // Wait for every prologue load here: the loop below then only waits for
   // the prefetch it issues itself (without this, the wait inserted for the
@@ -278,17 +293,10 @@ __global__ __launch_bounds__(WAVES * kWave) void bgemm_kernel(const T *At, long 
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   long c = c0;
-  for (; c + PF + 1 <= c1; c += PF + 1) {
-    step(Stage<0>{}, c);
-    step(Stage<1>{}, c + 1);
-    if constexpr (PF >= 2) step(Stage<2>{}, c + 2);
-    if constexpr (PF >= 3) step(Stage<3>{}, c + 3);
-  }
-  if (c < c1) step(Stage<0>{}, c);
-  if constexpr (PF >= 2)
-    if (c + 1 < c1) step(Stage<1>{}, c + 1);
-  if constexpr (PF >= 3)
-    if (c + 2 < c1) step(Stage<2>{}, c + 2);
+  for (; c + U <= c1; c += U) static_for<0, U>([&](auto i) { step(i, c + decltype(i)::value); });
+  static_for<0, U - 1>([&](auto i) {
+    if (c + decltype(i)::value < c1) step(i, c + decltype(i)::value);
+  });
 
   float *Pk = P + (size_t)slab * mpad * ldn;
 #pragma unroll
@@ -814,7 +822,7 @@ struct kmt_batch {
   int begun = 0;
   // tiling (shared) / rows (grouped)
   int tile_var = 3, ksplit = 8, lo = 1;
-  double tune_ms[30] = {0};  // (tile_var 0..5) x (split in 1,2,4,8,16)
+  double tune_ms[96] = {0};  // (tile_var 0..5) x (split 1..16)
   hipGraphExec_t g1 = nullptr, g8 = nullptr;
   // proposer
   uint16_t *mlp_w = nullptr;
@@ -1031,21 +1039,21 @@ struct kmt_batch {
   // per workgroup), 1 = 8 waves x 1 band (256), 2 = 8 waves x 2 bands (512),
   // 3 / 4 = 8 waves x 1 band with two / three chunks of A in flight,
   // 5 = 4 waves x 1 band with two chunks in flight
-  template <class T, int MT, int NT, int WAVES, bool LO, int PF> void gemm_v() {
+  template <class T, int MT, int NT, int WAVES, bool LO, int PF, int VD> void gemm_v() {
     const long nkc = lda_shared / kKC;
     const long slab_chunks = (nkc + ksplit - 1) / ksplit;
     const long nrb = rows_shared / (32L * NT * WAVES);
-    bgemm_kernel<T, MT, NT, WAVES, LO, PF><<<(unsigned)(nrb * ksplit), WAVES * kWave, 0, stream>>>(
+    bgemm_kernel<T, MT, NT, WAVES, LO, PF, VD><<<(unsigned)(nrb * ksplit), WAVES * kWave, 0, stream>>>(
         (const T *)A, nkc, vthi, vtlo, lda_shared, ksplit, slab_chunks, P, ldn, mpad);
   }
   template <class T, int MT, bool LO> void gemm_lo() {
     switch (tile_var) {
-      case 0: gemm_v<T, MT, 1, 4, LO, 1>(); break;
-      case 1: gemm_v<T, MT, 1, 8, LO, 1>(); break;
-      case 3: gemm_v<T, MT, 1, 8, LO, 2>(); break;
-      case 4: gemm_v<T, MT, 1, 8, LO, 3>(); break;
-      case 5: gemm_v<T, MT, 1, 4, LO, 2>(); break;
-      default: gemm_v<T, MT, 2, 8, LO, 1>(); break;
+      case 0: gemm_v<T, MT, 1, 4, LO, 1, 2>(); break;
+      case 1: gemm_v<T, MT, 1, 8, LO, 1, 2>(); break;
+      case 3: gemm_v<T, MT, 1, 8, LO, 2, 2>(); break;
+      case 4: gemm_v<T, MT, 1, 8, LO, 3, 2>(); break;
+      case 5: gemm_v<T, MT, 1, 4, LO, 2, 1>(); break;
+      default: gemm_v<T, MT, 2, 8, LO, 1, 1>(); break;
     }
   }
   template <class T, int MT> void gemm_mt() {
@@ -1356,10 +1364,14 @@ struct kmt_batch {
     return KMT_OK;
   }
 
-  // Tile autotuner of the shared (MFMA) product: the five workgroup shapes
-  // (tile_var) x split-K {1, 2, 4, 8, 16}.  The split changes the order in
-  // which partial sums are added (results agree to FP32 rounding); the
-  // workgroup shape does not change results.  Each candidate is scored by
+  // Tile autotuner of the shared (MFMA) product: the six workgroup shapes
+  // (tile_var) x split-K 1..16.  The split changes the order in which
+  // partial sums are added (results agree to FP32 rounding); the workgroup
+  // shape does not change results.  Splits that are not powers of two
+  // matter: the grid is (row blocks x split) workgroups, and the split that
+  // makes it just fill a whole number of rounds of resident workgroups
+  // leaves the least idle tail (at B = 64, N = 20000, 8-wave shapes: split
+  // 6, 474 workgroups, beats split 8, 632).  Each candidate is scored by
   // its measured time plus the time to move its split-K partials (written
   // here, read back by the epilogue: 2 x ksplit x mpad x ldn x 4 bytes at
   // 6 TB/s), so a wider split has to pay for its extra traffic.
@@ -1372,12 +1384,11 @@ struct kmt_batch {
     HIP_TRY(hipEventCreate(&e1));
     double best = 1e30;
     int bnt = 3, bks = 8;
-    const int splits[5] = {1, 2, 4, 8, 16};
     int idx = 0;
     for (int nt = 0; nt < 6; ++nt)
-      for (int si = 0; si < 5; ++si, ++idx) {
+      for (int si = 1; si <= 16; ++si, ++idx) {
         tile_var = nt;
-        ksplit = splits[si];
+        ksplit = si;
         gemm_t_any();
         HIP_TRY(hipEventRecord(e0, stream));
         for (int r = 0; r < reps; ++r) gemm_t_any();
@@ -1397,14 +1408,13 @@ struct kmt_batch {
     drop_graphs();
     if (best_nt) *best_nt = bnt;
     if (best_split) *best_split = bks;
-    if (best_ms) *best_ms = tune_ms[bnt * 5 + (bks == 1 ? 0 : bks == 2 ? 1 : bks == 4 ? 2 : bks == 8 ? 3 : 4)];
+    if (best_ms) *best_ms = tune_ms[bnt * 16 + bks - 1];
     return KMT_OK;
   }
   int select_tile(int variant, int split) {
     if (!shared) return kmt_fail(KMT_ERR_STATE, "tile selection applies to shared (MFMA) batches");
     if (variant < 0 || variant > 5) return kmt_fail(KMT_ERR_ARG, "workgroup variant must be 0..5 (got %d)", variant);
-    if (split != 1 && split != 2 && split != 4 && split != 8 && split != 16)
-      return kmt_fail(KMT_ERR_ARG, "split_k must be 1, 2, 4, 8 or 16 (got %d)", split);
+    if (split < 1 || split > 16) return kmt_fail(KMT_ERR_ARG, "split_k must be 1..16 (got %d)", split);
     tile_var = variant;
     ksplit = split;
     drop_graphs();
@@ -1540,9 +1550,9 @@ int kmt_batch_select_tile(kmt_batch *bp, int variant, int split_k) {
   if (!bp) return kmt_fail(KMT_ERR_ARG, "batch is NULL");
   return bp->select_tile(variant, split_k);
 }
-int kmt_batch_tune_table(kmt_batch *bp, double *ms30) {
+int kmt_batch_tune_table(kmt_batch *bp, double *ms96) {
   B_CHECK;
-  if (ms30) for (int q = 0; q < 30; ++q) ms30[q] = bp->tune_ms[q];
+  if (ms96) for (int q = 0; q < 96; ++q) ms96[q] = bp->tune_ms[q];
   return KMT_OK;
 }
 int kmt_batch_set_vector_split(kmt_batch *bp, int hi_lo) {
