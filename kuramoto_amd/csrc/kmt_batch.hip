@@ -1379,6 +1379,18 @@ struct kmt_batch {
     if (!shared) return KMT_OK;
     if (!A) return kmt_fail(KMT_ERR_STATE, "no adjacency");
     KMT_TRY(ensure_P());
+    if (!begun) {
+      // Time the candidates on the vectors of the instances' initial phases,
+      // not on the zeroed buffers: the MFMA pipes draw less power on zero
+      // operands and then run at higher clocks, which makes every candidate
+      // look faster than it is inside a run.  (After kmt_batch_begin the
+      // buffers hold the current stage's vectors; begin rewrites them.)
+      HIP_TRY(hipMemcpyAsync(inst, host_inst.data(), sizeof(BInst) * B, hipMemcpyHostToDevice, stream));
+      dim3 g2((unsigned)nblk, (unsigned)B);
+      if (storage == KMT_ADJ_F16) bsincos_kernel<f16_t><<<g2, kBlock, 0, stream>>>(y, inst, ldn, cs[1], vt_out(), 1);
+      else bsincos_kernel<bf16_t><<<g2, kBlock, 0, stream>>>(y, inst, ldn, cs[1], vt_out(), 1);
+      HIP_TRY(hipGetLastError());
+    }
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
