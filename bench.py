@@ -133,6 +133,37 @@ def batch_to_time(name, proposer, t_end):
                 osc_steps=st.osc_steps, proposer_accepts=st.proposer_accepts, proposer_rejects=st.proposer_rejects)
 
 
+def sequential_plain_to_time(name, t_end):
+    """The shared batch's instances run one after another as plain FP32
+    runs with the dense FP32 adjacency (the reference's own path), each from
+    t=0 to t_end: host wall clock around each run's integration loop (the
+    adjacency is generated before the clock starts, as for the batch)."""
+    import kuramoto_amd as km
+
+    cfg = BATCH_CONFIGS[name]
+    N = cfg["Ns"][0]
+    X = km.factor(N, 5, 1, gaussian=True).astype(np.float32)
+    w = km.factor_weights(km.F32, N, 5, gaussian=True)
+    dt = np.float32(cfg["dt"])
+    te = (dt * np.arange(1, int(np.float32(t_end) / dt) + 1)).astype(np.float64)
+    secs, rs, tot = 0.0, [], dict(rhs_evals=0, accepted=0, rejected=0)
+    for b in range(cfg["B"]):
+        th, om = km.draws(km.F32, N, cfg["seeds"][b], normal=cfg["normal"][b], init=6.283185307)
+        with km.Integrator(N, km.F32, atol=1e-6, c1=float(cfg["cs"][b])) as it:
+            it.set_omega(om)
+            it.set_factor(X, w)
+            it.build_adjacency()
+            it.set_state(th, 0.0, 0.01)
+            t0 = time.perf_counter()
+            res = it.run(te, t_end, keep_rows=False, chunk=64)
+            secs += time.perf_counter() - t0
+        st = res["status"]
+        for k in tot:
+            tot[k] += getattr(st, k)
+        rs.append(res["r"])
+    return dict(seconds=secs, r=rs, **tot)
+
+
 def batch_run(name, steps, warmup, proposer):
     cfg = BATCH_CONFIGS[name]
     bt, Ns, tune = make_batch(name, proposer)
@@ -186,6 +217,24 @@ def batch_main(args):
                  "batched_ms_per_attempt": per_attempt_batched,
                  "speedup_vs_64_sequential_plain": 64 * seq_ms / per_attempt_batched,
                  "tile": {"workgroup_variant": r["tune"][0], "split_k": r["tune"][1]}}
+        # time to solution: every instance integrated from t=0 to t_end by the
+        # batch (standard controller and learned proposer) and by 64 plain
+        # runs one after another
+        t_end = args.batch_t_end
+        runs = {m: batch_to_time(args.config, m == "batched_learned_proposer", t_end)
+                for m in ("batched_standard_controller", "batched_learned_proposer")}
+        runs["sequential_plain_fp32"] = sequential_plain_to_time(args.config, t_end)
+        plain_r = runs["sequential_plain_fp32"]["r"]
+        dr = {m: max(float(np.max(np.abs(a - b))) for a, b in zip(runs[m]["r"], plain_r))
+              for m in ("batched_standard_controller", "batched_learned_proposer")}
+        for m in runs:
+            runs[m].pop("r")
+        seq_s = runs["sequential_plain_fp32"]["seconds"]
+        extra.update({f"whole_run_to_t{t_end:g}": runs,
+                      "max_abs_dr_vs_sequential_plain": dr,
+                      "time_to_solution_speedup_vs_sequential_plain": {
+                          m: seq_s / runs[m]["seconds"]
+                          for m in ("batched_standard_controller", "batched_learned_proposer")}})
     else:
         alg = sum(2 * n * n + 3 * 4 * n for n in Ns)
         # the learned proposer changes how many attempts reach a given t, so
@@ -231,7 +280,8 @@ def main():
     ap.add_argument("--config", default="fp64-n50000", choices=sorted(CONFIGS) + sorted(BATCH_CONFIGS))
     ap.add_argument("--cpu-steps", type=int, default=20, help="accepted steps of the CPU baseline sample (0: skip)")
     ap.add_argument("--batch-t-end", type=float, default=10.0,
-                    help="uneven batch: end time of the whole runs that compare the two step controllers")
+                    help="batched workloads: end time of the whole runs that compare the step controllers "
+                         "(and, for the shared batch, 64 sequential plain runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (the row-sharded "

@@ -256,6 +256,14 @@ int kmt_batch_tune_table(kmt_batch *batch, double *ms96);
 /* pin the shared product's workgroup variant and split-K instead of the
  * autotuner's choice */
 int kmt_batch_select_tile(kmt_batch *batch, int variant, int split_k);
+/* shared mode: once finished instances leave the active ones fitting in
+ * fewer 32-row MFMA tiles, kmt_batch_advance renumbers the active instances'
+ * rows and runs the product on those rows only (default on; results are
+ * bit-identical either way).  kmt_batch_product_rows reports the rows the
+ * product currently computes (2B in grouped mode) and how many times the
+ * rows were compacted since kmt_batch_create. */
+int kmt_batch_set_compaction(kmt_batch *batch, int enabled);
+int kmt_batch_product_rows(kmt_batch *batch, int *rows, int *compactions);
 /* HIP-event timing of `attempts` attempts; coupling_ms = average of the
  * coupling launches (MFMA product or grouped gemv) inside the region */
 int kmt_batch_time_attempts(kmt_batch *batch, int attempts, double *region_ms, double *coupling_ms,

@@ -106,6 +106,8 @@ _SIGNATURES = {
     "kmt_batch_time_attempts": ([_P, _I, _P, _P, _P], _I),
     "kmt_batch_set_vector_split": ([_P, _I], _I),
     "kmt_batch_tune_table": ([_P, _P], _I),
+    "kmt_batch_set_compaction": ([_P, _I], _I),
+    "kmt_batch_product_rows": ([_P, _P, _P], _I),
     "kmt_batch_select_tile": ([_P, _I, _I], _I),
     "kmt_batch_set_proposer": ([_P, _P, _I, _I], _I),
     "kmt_proposer_param_count": ([], _I),
@@ -493,6 +495,16 @@ class Batch:
 
     def select_tile(self, variant, split_k):
         _check(self.lib.kmt_batch_select_tile(self.ptr, variant, split_k), "kmt_batch_select_tile")
+
+    def set_compaction(self, on):
+        _check(self.lib.kmt_batch_set_compaction(self.ptr, 1 if on else 0), "kmt_batch_set_compaction")
+
+    def product_rows(self):
+        """(rows the shared product computes now, compactions so far)."""
+        rows, n = ctypes.c_int(), ctypes.c_int()
+        _check(self.lib.kmt_batch_product_rows(self.ptr, ctypes.byref(rows), ctypes.byref(n)),
+               "kmt_batch_product_rows")
+        return rows.value, n.value
 
     def tune_table(self):
         """ms per product, [workgroup shape 0..5][split-K 1..16]."""

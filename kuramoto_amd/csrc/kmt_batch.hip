@@ -56,6 +56,7 @@ struct BInst {
   float r_now, r_prev;       // order parameter at the last two accepted steps
   float t_prev;
   int hist_n;
+  int vrow;             // shared mode: row of cos in V^T and in the product (sin: vrow + half); -1 = none
 };
 
 // ---------------------------------------------------------------------
@@ -69,17 +70,18 @@ template <> __device__ __forceinline__ float from16<bf16_t>(uint16_t u) { return
 template <> __device__ __forceinline__ float from16<f16_t>(uint16_t u) { return (float)__builtin_bit_cast(_Float16, u); }
 
 struct VtOut {
-  uint16_t *hi, *lo;  // [Mpad][ldk], row b = cos_b, row B + b = sin_b
+  uint16_t *hi, *lo;  // [Mpad][ldk], row vrow = cos, row half + vrow = sin of one instance
   long ldk;
-  int B;
+  int half;
 };
 
-template <class T> __device__ __forceinline__ void put_vt(const VtOut &v, int b, long n, float c, float s) {
+template <class T> __device__ __forceinline__ void put_vt(const VtOut &v, int row, long n, float c, float s) {
+  if (row < 0) return;
   const uint16_t ch = to16<T>(c), sh = to16<T>(s);
-  v.hi[(size_t)b * v.ldk + n] = ch;
-  v.hi[(size_t)(v.B + b) * v.ldk + n] = sh;
-  v.lo[(size_t)b * v.ldk + n] = to16<T>(c - from16<T>(ch));
-  v.lo[(size_t)(v.B + b) * v.ldk + n] = to16<T>(s - from16<T>(sh));
+  v.hi[(size_t)row * v.ldk + n] = ch;
+  v.hi[(size_t)(v.half + row) * v.ldk + n] = sh;
+  v.lo[(size_t)row * v.ldk + n] = to16<T>(c - from16<T>(ch));
+  v.lo[(size_t)(v.half + row) * v.ldk + n] = to16<T>(s - from16<T>(sh));
 }
 
 template <class T>
@@ -92,7 +94,18 @@ __global__ __launch_bounds__(kBlock) void bsincos_kernel(const float *y, const B
   const float c = cosf(v), s = sinf(v);
   cs[((size_t)b * 2) * ldn + n] = c;
   cs[((size_t)b * 2 + 1) * ldn + n] = s;
-  if (write_vt) put_vt<T>(vt, b, n, c, s);
+  if (write_vt) put_vt<T>(vt, inst[b].vrow, n, c, s);
+}
+
+// Rewrite the V^T rows of every active instance from its float (cos, sin)
+// stage vector (after the rows were reassigned): the same conversion as
+// put_vt at the vector's producer, so the product sees identical bits.
+template <class T>
+__global__ __launch_bounds__(kBlock) void bvt_rows_kernel(const float *cs, const BInst *inst, long ldn, VtOut vt) {
+  const int b = blockIdx.y;
+  const long n = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (!inst[b].active || n >= inst[b].n) return;
+  put_vt<T>(vt, inst[b].vrow, n, cs[((size_t)b * 2) * ldn + n], cs[((size_t)b * 2 + 1) * ldn + n]);
 }
 
 // ---------------------------------------------------------------------
@@ -454,6 +467,7 @@ struct BVecs {
   float *f_out;       // RHS kind output, [B][ldn]
   const float *P;     // [ksplit][mrows][ldn]
   int ksplit, mrows;
+  int phalf;          // row of an instance's S = its row of C + phalf
   BInst *inst;
   double *err_part;   // [B][nblk]
   int nblk;
@@ -475,8 +489,8 @@ __global__ __launch_bounds__(kBlock) void bepilogue_kernel(const BVecs a) {
   if (n < in->n) {
     float C = 0.f, S = 0.f;
     for (int ks = 0; ks < a.ksplit; ++ks) {
-      C += a.P[((size_t)ks * a.mrows + b) * a.ldn + n];
-      S += a.P[((size_t)ks * a.mrows + a.B + b) * a.ldn + n];
+      C += a.P[((size_t)ks * a.mrows + in->vrow) * a.ldn + n];
+      S += a.P[((size_t)ks * a.mrows + a.phalf + in->vrow) * a.ldn + n];
     }
     C *= in->mscale;
     S *= in->mscale;
@@ -499,7 +513,7 @@ __global__ __launch_bounds__(kBlock) void bepilogue_kernel(const BVecs a) {
       const float c = cosf(yt), si = sinf(yt);
       a.cs_out[2 * vb + n] = c;
       a.cs_out[2 * vb + a.ldn + n] = si;
-      if (a.write_vt) put_vt<T>(a.vt, b, n, c, si);
+      if (a.write_vt) put_vt<T>(a.vt, in->vrow, n, c, si);
     } else {
       a.kbuf[6 * kst + vb + n] = f;
       float acc = a.kbuf[vb + n] * (float)tab::E[0];
@@ -696,7 +710,7 @@ __global__ __launch_bounds__(kBlock) void bemit_kernel(const BEmit a) {
     const float c = cosf(yt), s = sinf(yt);
     a.cs_out[2 * vb + n] = c;
     a.cs_out[2 * vb + a.ldn + n] = s;
-    if (a.write_vt) put_vt<T>(a.vt, b, n, c, s);
+    if (a.write_vt) put_vt<T>(a.vt, in->vrow, n, c, s);
   };
   if (!in->commit_pending) {
     if (valid) stage2(a.y[vb + n], a.kbuf[vb + n]);
@@ -799,6 +813,12 @@ struct kmt_batch {
   std::vector<long> Nb;
   long Nmax = 0, ldn = 0, lda_shared = 0, rows_shared = 0;
   int mpad = 32, mt = 1, nblk = 0;
+  // shared mode: the product runs on the rows of the active instances only.
+  // Instance b's cos sits in row inst[b].vrow of V^T and of the product, its
+  // sin in row vhalf + vrow; mt_act 32-row tiles (mpad_act rows) are used.
+  // kmt_batch_begin starts from vrow = b, vhalf = B; when finished instances
+  // leave fewer tiles' worth of rows, compact() renumbers the active ones.
+  int mt_act = 1, mpad_act = 32, vhalf = 0, compaction = 1;
   hipStream_t stream = nullptr;
   float *y = nullptr, *ylast = nullptr, *ynew = nullptr, *om = nullptr, *kbuf = nullptr, *cs[2] = {nullptr, nullptr};
   float *P = nullptr;
@@ -886,12 +906,16 @@ struct kmt_batch {
       std::memset(&host_inst[b], 0, sizeof(BInst));
       host_inst[b].n = (int)Nb[b];
       host_inst[b].mscale = 1.f;
+      host_inst[b].vrow = b;
     }
     mscale.assign(B, 1.f);
     if (shared) {
       mt = (2 * B + 31) / 32;
       if (mt == 3) mt = 4;
       mpad = 32 * mt;
+      mt_act = mt;
+      mpad_act = mpad;
+      vhalf = B;
       lda_shared = rup(Nmax, 64);
       rows_shared = ldn;  // multiple of 512: whole workgroups of every product variant
       KMT_TRY(alloc(&vthi, (size_t)mpad * lda_shared));
@@ -1030,7 +1054,7 @@ struct kmt_batch {
     v.hi = vthi;
     v.lo = vtlo;
     v.ldk = lda_shared;
-    v.B = B;
+    v.half = vhalf;
     return v;
   }
 
@@ -1044,7 +1068,7 @@ struct kmt_batch {
     const long slab_chunks = (nkc + ksplit - 1) / ksplit;
     const long nrb = rows_shared / (32L * NT * WAVES);
     bgemm_kernel<T, MT, NT, WAVES, LO, PF, VD><<<(unsigned)(nrb * ksplit), WAVES * kWave, 0, stream>>>(
-        (const T *)A, nkc, vthi, vtlo, lda_shared, ksplit, slab_chunks, P, ldn, mpad);
+        (const T *)A, nkc, vthi, vtlo, lda_shared, ksplit, slab_chunks, P, ldn, mpad_act);
   }
   template <class T, int MT, bool LO> void gemm_lo() {
     switch (tile_var) {
@@ -1060,8 +1084,8 @@ struct kmt_batch {
     if (lo) gemm_lo<T, MT, true>(); else gemm_lo<T, MT, false>();
   }
   template <class T> void gemm_t() {
-    if (mt == 1) gemm_mt<T, 1>();
-    else if (mt == 2) gemm_mt<T, 2>();
+    if (mt_act == 1) gemm_mt<T, 1>();
+    else if (mt_act == 2) gemm_mt<T, 2>();
     else gemm_mt<T, 4>();
   }
   void coupling(int check) {
@@ -1091,7 +1115,8 @@ struct kmt_batch {
     v.f_out = kbuf;  // the RHS kind writes k1
     v.P = P;
     v.ksplit = shared ? ksplit : 1;
-    v.mrows = shared ? mpad : 2 * B;
+    v.mrows = shared ? mpad_act : 2 * B;
+    v.phalf = shared ? vhalf : B;
     v.inst = inst;
     v.err_part = err_part;
     v.nblk = nblk;
@@ -1235,6 +1260,12 @@ struct kmt_batch {
       in.active = t0 < t1 ? 1 : 0;
       in.rhs_evals = 1;
       in.last_accept = 1;
+      in.vrow = b;
+    }
+    if (shared) {
+      mt_act = mt;
+      mpad_act = mpad;
+      vhalf = B;
     }
     HIP_TRY(hipMemcpyAsync(inst, host_inst.data(), sizeof(BInst) * B, hipMemcpyHostToDevice, stream));
     // k1 = f(y) for every instance, then the first stage-2 vectors
@@ -1283,6 +1314,7 @@ struct kmt_batch {
     KMT_TRY(fetch());
     int any = 0;
     for (const BInst &in : host_inst) any |= in.active;
+    if (any && n > 0 && shared && compaction) KMT_TRY(compact());
     if (any && n > 0) {
       if (!g1) KMT_TRY(build_graphs());
       for (; n >= 8; n -= 8) HIP_TRY(hipGraphLaunch(g8, stream));
@@ -1292,6 +1324,38 @@ struct kmt_batch {
     fill(s);
     return KMT_OK;
   }
+
+  // Renumber the active instances' rows when they fit in fewer 32-row tiles
+  // (called between host synchronisations, nothing in flight).  Each output
+  // row of the product depends only on its own row of V^T, so results are
+  // bit-identical to the uncompacted run; only the finished instances' rows
+  // stop being multiplied.  Graphs are rebuilt for the smaller product.
+  int compact() {
+    int act = 0;
+    for (const BInst &in : host_inst) act += in.active ? 1 : 0;
+    int need = std::max(1, (2 * act + 31) / 32);
+    if (need == 3) need = 4;
+    if (act == 0 || need >= mt_act) return KMT_OK;
+    int row = 0;
+    for (int b = 0; b < B; ++b) {
+      host_inst[b].vrow = host_inst[b].active ? row++ : -1;
+      HIP_TRY(hipMemcpyAsync(&inst[b].vrow, &host_inst[b].vrow, sizeof(int), hipMemcpyHostToDevice, stream));
+    }
+    vhalf = act;
+    mt_act = need;
+    mpad_act = 32 * need;
+    // the next attempt's first product reads the stage-2 vectors, which
+    // cs[0] holds in FP32: write them into the new rows
+    dim3 g2((unsigned)nblk, (unsigned)B);
+    if (storage == KMT_ADJ_F16) bvt_rows_kernel<f16_t><<<g2, kBlock, 0, stream>>>(cs[0], inst, ldn, vt_out());
+    else bvt_rows_kernel<bf16_t><<<g2, kBlock, 0, stream>>>(cs[0], inst, ldn, vt_out());
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(stream));
+    drop_graphs();
+    ++compactions;
+    return KMT_OK;
+  }
+  int compactions = 0;
 
   int read_order(int b, int max_n, float *r, int *nout) {
     if (b < 0 || b >= B) return kmt_fail(KMT_ERR_ARG, "instance out of range");
@@ -1409,7 +1473,7 @@ struct kmt_batch {
         float ms = 0;
         HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
         tune_ms[idx] = ms / reps;
-        const double partial_ms = 2.0 * ksplit * mpad * (double)ldn * 4.0 / 6e12 * 1e3;
+        const double partial_ms = 2.0 * ksplit * mpad_act * (double)ldn * 4.0 / 6e12 * 1e3;
         const double score = ms / reps + partial_ms;
         if (score < best) { best = score; bnt = nt; bks = ksplit; }
       }
@@ -1561,6 +1625,17 @@ int kmt_batch_time_attempts(kmt_batch *bp, int attempts, double *region_ms, doub
 int kmt_batch_select_tile(kmt_batch *bp, int variant, int split_k) {
   if (!bp) return kmt_fail(KMT_ERR_ARG, "batch is NULL");
   return bp->select_tile(variant, split_k);
+}
+int kmt_batch_set_compaction(kmt_batch *bp, int enabled) {
+  B_CHECK;
+  bp->compaction = enabled ? 1 : 0;
+  return KMT_OK;
+}
+int kmt_batch_product_rows(kmt_batch *bp, int *rows, int *compactions) {
+  B_CHECK;
+  if (rows) *rows = bp->shared ? bp->mpad_act : 2 * bp->B;
+  if (compactions) *compactions = bp->compactions;
+  return KMT_OK;
 }
 int kmt_batch_tune_table(kmt_batch *bp, double *ms96) {
   B_CHECK;

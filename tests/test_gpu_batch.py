@@ -180,3 +180,39 @@ def test_shared_workgroup_variants_agree(km, storage):
         for b in range(B):
             assert np.array_equal(out[var][0]["r"][b], out[0][0]["r"][b]), (var, b)
             assert np.array_equal(out[var][1][b][0], out[0][1][b][0]), (var, b)
+
+
+@pytest.mark.parametrize("storage", ["bf16", "f16"])
+def test_shared_row_compaction_is_bit_identical(km, storage):
+    """Instances with Cauchy frequencies need many more steps than those with
+    normal ones; once only the former are left (4 of 20 -> 8 rows), the
+    shared product runs on one 32-row tile instead of two.  Each output row
+    depends only on its own row of V^T, so the run equals the uncompacted
+    one bit for bit (same workgroup variant and split)."""
+    B, N = 20, 1000
+    X = km.factor(N, 5, 21, gaussian=True).astype(np.float32)
+    w = km.factor_weights(km.F32, N, 5, gaussian=True)
+    te = _grid(2.0, 0.25)
+    out = {}
+    for on in (True, False):
+        print(f"case compaction={on} {storage}", flush=True)
+        with km.Batch([N] * B, atol=1e-6, shared=True, storage=storage) as bt:
+            for b in range(B):
+                th, om = km.draws(km.F32, N, 60 + b, normal=b >= 4, init=6.283185307)
+                bt.set_instance(b, om, th, 0.5 + 0.15 * b, 0.0, 0.01)
+            bt.set_factor(-1, X, w)
+            bt.select_tile(3, 4)
+            bt.set_compaction(on)
+            assert bt.product_rows() == (64, 0)
+            res = bt.run(te, 2.0, chunk=8)
+            out[on] = (res, [bt.get_instance(b) for b in range(B)], bt.product_rows())
+    res_on, fin_on, rows_on = out[True]
+    res_off, fin_off, rows_off = out[False]
+    acc = [f[1]["accepted"] for f in fin_off]
+    assert min(acc[:4]) > 2 * max(acc[4:])  # the Cauchy instances outlast the others
+    assert rows_on[0] == 32 and rows_on[1] >= 1 and rows_off == (64, 0)
+    assert res_on["status"].active == 0 and res_on["status"].errors == 0
+    for b in range(B):
+        assert np.array_equal(res_on["r"][b], res_off["r"][b]), b
+        assert np.array_equal(fin_on[b][0], fin_off[b][0]), b
+        assert fin_on[b][1] == fin_off[b][1], b
