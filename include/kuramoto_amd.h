@@ -174,9 +174,11 @@ int kmt_time_attempts(kmt_ctx *ctx, int attempts, double *region_ms, double *sta
 
 /* Measured bandwidth ceiling of the device: a plain 16-byte copy kernel
  * (reads + writes `bytes` each way) and a read-only stream of `bytes`,
- * `reps` launches each, timed with HIP events.  GB/s (1e9 B/s).  No
- * reference counterpart: the denominator of the roofline next to the
- * nominal peak, and the calibration kernel of the HBM byte counters. */
+ * `reps` launches each in three launch shapes (4 or 8 loads in flight per
+ * lane; 8, 16 or 32 workgroups per CU), timed with HIP events; the best
+ * shape's GB/s (1e9 B/s) is reported.  No reference counterpart: the
+ * denominator of the roofline next to the nominal peak, and the
+ * calibration kernel of the HBM byte counters. */
 int kmt_bandwidth_probe(int device, size_t bytes, int reps, double *copy_gbps, double *read_gbps);
 
 /* Autotune the dense stage kernel on this context's adjacency: rows per
