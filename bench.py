@@ -51,6 +51,15 @@ def profiled_traffic(config, kernel="stage_kernel"):
     return None
 
 
+def timed_advance(run, steps):
+    """The timed region: exactly `steps` attempts through the production path
+    (replayed hipGraphs of 8 and 1 attempts), host clock from a synchronized
+    start to the status read-back that waits for the last attempt."""
+    t0 = time.perf_counter()
+    st = run.advance(steps)
+    return (time.perf_counter() - t0) * 1e3, st
+
+
 def gpu_run(cfg, steps, warmup):
     import kuramoto_amd as km
 
@@ -80,7 +89,11 @@ def gpu_run(cfg, steps, warmup):
     it.read_times()
     it.read_outputs()
     st0 = it.advance(0)
-    region_ms, stage_ms, st = it.time_attempts(steps)
+    region_ms, st = timed_advance(it, steps)
+    # per-launch kernel time for the roofline: HIP events around every stage
+    # launch of further attempts, after the timed region (the event markers
+    # add a few microseconds between launches, so they stay out of it)
+    _, stage_ms, _ = it.time_attempts(min(steps, 10))
     lda, eb, rows, nt, bal = it.layout()
     it.close()
     copy_gbps, read_gbps = km.bandwidth_probe(8 << 30, 5)
@@ -173,7 +186,8 @@ def batch_run(name, steps, warmup, proposer):
     if warmup:
         bt.advance(warmup)
     st0 = bt.advance(0)
-    region_ms, coup_ms, st = bt.time_attempts(steps)
+    region_ms, st = timed_advance(bt, steps)
+    _, coup_ms, _ = bt.time_attempts(min(steps, 10))
     bt.close()
     return dict(Ns=Ns, region_ms=region_ms, coupling_ms=coup_ms, st0=st0, st=st, tune=tune)
 
@@ -191,7 +205,7 @@ def plain_attempt_ms(N, steps=10):
         it.set_state(th, 0.0, 0.01)
         it.begin(np.arange(1, 51, dtype=np.float64), 50.0, keep_rows=False, ring_rows=64)
         it.advance(2)
-        region_ms, _, _ = it.time_attempts(steps)
+        region_ms, _ = timed_advance(it, steps)
     return region_ms / steps
 
 
