@@ -76,3 +76,6 @@ def test_errors_are_reported(km):
     assert "N must be >= 1" in str(e.value)
     with pytest.raises(km.KmtError):
         km.draws(7, 10, 1)
+    with pytest.raises(km.KmtError) as e:
+        km.bandwidth_probe(1000, 1)
+    assert ">= 1 MiB" in str(e.value)
