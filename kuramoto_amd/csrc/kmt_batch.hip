@@ -910,8 +910,7 @@ struct kmt_batch {
     }
     mscale.assign(B, 1.f);
     if (shared) {
-      mt = (2 * B + 31) / 32;
-      if (mt == 3) mt = 4;
+      mt = (2 * B + 31) / 32;  // 32-row MFMA tiles of V^T: 1..4
       mpad = 32 * mt;
       mt_act = mt;
       mpad_act = mpad;
@@ -1086,6 +1085,7 @@ struct kmt_batch {
   template <class T> void gemm_t() {
     if (mt_act == 1) gemm_mt<T, 1>();
     else if (mt_act == 2) gemm_mt<T, 2>();
+    else if (mt_act == 3) gemm_mt<T, 3>();
     else gemm_mt<T, 4>();
   }
   void coupling(int check) {
@@ -1333,8 +1333,7 @@ struct kmt_batch {
   int compact() {
     int act = 0;
     for (const BInst &in : host_inst) act += in.active ? 1 : 0;
-    int need = std::max(1, (2 * act + 31) / 32);
-    if (need == 3) need = 4;
+    const int need = std::max(1, (2 * act + 31) / 32);
     if (act == 0 || need >= mt_act) return KMT_OK;
     int row = 0;
     for (int b = 0; b < B; ++b) {

@@ -54,7 +54,7 @@ def test_grouped_equals_single_runs(km, storage):
         assert np.float32(finals[b][1]["t"]) == np.float32(t)
 
 
-@pytest.mark.parametrize("B,N", [(3, 1000), (64, 2048)])
+@pytest.mark.parametrize("B,N", [(3, 1000), (40, 1000), (64, 2048)])
 @pytest.mark.parametrize("storage", ["bf16", "f16"])
 @pytest.mark.parametrize("normal", [True, False])
 def test_shared_mfma_matches_single_runs(km, orc, B, N, storage, normal):
@@ -84,7 +84,7 @@ def test_shared_mfma_matches_single_runs(km, orc, B, N, storage, normal):
         res = bt.run(te, t1, ring_rows=64, chunk=8)
         finals = [bt.get_instance(b) for b in range(B)]
     assert res["status"].active == 0 and res["status"].errors == 0
-    check = range(B) if B <= 4 else [0, 17, 40, 63]
+    check = range(B) if B <= 4 else [0, B // 4 + 1, B // 2 + 8, B - 1]
     for b in check:
         with km.Integrator(N, km.F32, atol=1e-6, c1=cs[b]) as it:
             it.set_omega(om_all[b])
@@ -156,12 +156,13 @@ def test_attempt_log_and_proposer_fallback(km):
         assert np.max(np.abs(runs["standard"]["r"][b] - runs["proposer"]["r"][b])) < 1e-3
 
 
-@pytest.mark.parametrize("storage", ["bf16", "f16"])
-def test_shared_workgroup_variants_agree(km, storage):
+@pytest.mark.parametrize("storage,B", [("bf16", 5), ("f16", 5), ("bf16", 40)])
+def test_shared_workgroup_variants_agree(km, storage, B):
     """The six workgroup shapes of the MFMA product sum every output in the
     same order (chunk, k-step, hi then lo) within a split-K slab: at a fixed
-    split they give bit-identical runs."""
-    B, N = 5, 1500
+    split they give bit-identical runs (B = 5: one 32-row tile of V^T,
+    B = 40: three)."""
+    N = 1500
     X = km.factor(N, 5, 9, gaussian=True).astype(np.float32)
     w = km.factor_weights(km.F32, N, 5, gaussian=True)
     te = _grid(1.0, 0.25)
@@ -182,14 +183,15 @@ def test_shared_workgroup_variants_agree(km, storage):
             assert np.array_equal(out[var][1][b][0], out[0][1][b][0]), (var, b)
 
 
-@pytest.mark.parametrize("storage", ["bf16", "f16"])
-def test_shared_row_compaction_is_bit_identical(km, storage):
+@pytest.mark.parametrize("storage,B", [("bf16", 20), ("f16", 20), ("bf16", 48)])
+def test_shared_row_compaction_is_bit_identical(km, storage, B):
     """Instances with Cauchy frequencies need many more steps than those with
-    normal ones; once only the former are left (4 of 20 -> 8 rows), the
-    shared product runs on one 32-row tile instead of two.  Each output row
-    depends only on its own row of V^T, so the run equals the uncompacted
-    one bit for bit (same workgroup variant and split)."""
-    B, N = 20, 1000
+    normal ones; once only the former are left (4 of 20 or 48 -> 8 rows), the
+    shared product runs on one 32-row tile instead of two (B = 20) or three
+    (B = 48).  Each output row depends only on its own row of V^T, so the run
+    equals the uncompacted one bit for bit (same workgroup variant and
+    split)."""
+    N = 1000
     X = km.factor(N, 5, 21, gaussian=True).astype(np.float32)
     w = km.factor_weights(km.F32, N, 5, gaussian=True)
     te = _grid(2.0, 0.25)
@@ -203,14 +205,14 @@ def test_shared_row_compaction_is_bit_identical(km, storage):
             bt.set_factor(-1, X, w)
             bt.select_tile(3, 4)
             bt.set_compaction(on)
-            assert bt.product_rows() == (64, 0)
+            assert bt.product_rows() == (32 * ((2 * B + 31) // 32), 0)
             res = bt.run(te, 2.0, chunk=8)
             out[on] = (res, [bt.get_instance(b) for b in range(B)], bt.product_rows())
     res_on, fin_on, rows_on = out[True]
     res_off, fin_off, rows_off = out[False]
     acc = [f[1]["accepted"] for f in fin_off]
     assert min(acc[:4]) > 2 * max(acc[4:])  # the Cauchy instances outlast the others
-    assert rows_on[0] == 32 and rows_on[1] >= 1 and rows_off == (64, 0)
+    assert rows_on[0] == 32 and rows_on[1] >= 1 and rows_off == (32 * ((2 * B + 31) // 32), 0)
     assert res_on["status"].active == 0 and res_on["status"].errors == 0
     for b in range(B):
         assert np.array_equal(res_on["r"][b], res_off["r"][b]), b
