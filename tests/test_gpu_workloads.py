@@ -13,6 +13,9 @@
   on the uneven-size selection batch (N = 5000..40000, seeds 200..207, none
   of them in the training logs): fewer RHS evaluations than the standard controller on
   the same batch, every instance's r(t) within 1e-3 of the standard run.
+* The shared-adjacency batch at its full size (B = 64, N = 20000, BF16)
+  against plain FP32 runs of three of its instances on the same stored
+  matrix, with the row compaction exercised by the end of the run.
 """
 import os
 
@@ -94,3 +97,45 @@ def test_learned_proposer_saves_rhs_evaluations(km):
     for b in range(len(Ns)):
         assert len(runs["learned"]["r"][b]) == len(te)
         assert np.max(np.abs(runs["learned"]["r"][b] - runs["standard"]["r"][b])) < 1e-3, b
+
+
+def test_shared_batch64_n20000_matches_single_runs(km):
+    """The shared-adjacency batch at its full size (B = 64 sharing one
+    N = 20000 BF16 matrix, c_b = linspace(0.5, 4, 64), Cauchy frequencies):
+    instances 0, 31 and 63 against plain FP32 runs on the same stored
+    (dequantized) matrix to t = 2: r(t) within 1e-3, accepted steps of the
+    three together within 2% (each within 5%: with Cauchy frequencies the
+    FP32 error estimate of the fastest oscillators sits near its rounding
+    floor, so single counts move by a few steps between summation orders),
+    and the row compaction reached by the end of the batch run."""
+    from kuramoto_amd.workloads import make_batch
+
+    t1 = 2.0
+    te = _grid(t1, 1.0).astype(np.float64)
+    print("case batch64-n20000-bf16 full size vs single runs", flush=True)
+    bt, Ns, _ = make_batch("batch64-n20000-bf16", proposer=False)
+    try:
+        N = Ns[0]
+        Mq = bt.adjacency_rows(-1, 0, N)
+        res = bt.run(te, t1, ring_rows=64, chunk=8)
+        finals = [bt.get_instance(b)[1] for b in range(64)]
+        rows, compactions = bt.product_rows()
+    finally:
+        bt.close()
+    assert res["status"].active == 0 and res["status"].errors == 0
+    assert compactions >= 1 and rows < 128
+    cs = np.linspace(0.5, 4.0, 64)
+    tot_b = tot_s = 0
+    for b in (0, 31, 63):
+        th, om = km.draws(km.F32, N, b + 1, normal=False, init=6.283185307)
+        with km.Integrator(N, km.F32, atol=1e-6, c1=float(cs[b])) as it:
+            it.set_omega(om)
+            it.set_adjacency(Mq)
+            it.set_state(th, 0.0, 0.01)
+            single = it.run(te, t1, keep_rows=False)
+        acc = single["status"].accepted
+        assert abs(finals[b]["accepted"] - acc) <= max(2, int(0.05 * acc)), (b, finals[b]["accepted"], acc)
+        assert np.max(np.abs(res["r"][b] - single["r"])) < 1e-3, b
+        tot_b += finals[b]["accepted"]
+        tot_s += acc
+    assert abs(tot_b - tot_s) <= max(1, int(0.02 * tot_s)), (tot_b, tot_s)
