@@ -62,3 +62,34 @@ def test_n50000_rhs_rows_and_invariants(km, orc, name, row_tol, sum_tol, drift_t
     y64 = y.astype(np.float64)
     drift = y64.sum() - th64.sum() - t * om64.sum()
     assert abs(drift) <= drift_tol * (np.abs(y64).sum() + 1.0), drift
+
+
+def test_fp64_n50000_resume_is_bit_exact(km):
+    """Kill and resume at full size (the FILEBASEfs.dat restart, kuramoto.cu:471-494):
+    12 attempts in one context equal 5 attempts, a new context started from
+    kmt_get_state (y, t, h — the FILEBASEfs.dat contents), and 7 more, bit
+    for bit; the 20 GB adjacency is rebuilt by the new context."""
+    cfg = PLAIN["fp64-n50000"]
+    N, K, c1 = cfg["N"], cfg["K"], cfg["c1"]
+    th, om = km.draws(km.F64, N, cfg["seed"], normal=cfg["normal"], init=cfg["init"])
+    X = km.factor(N, K, cfg["seed"], gaussian=True)
+    w = km.factor_weights(km.F64, N, K, gaussian=True)
+    te = np.arange(1, 101, dtype=np.float64)
+
+    def run(y0, t0, h0, attempts):
+        with km.Integrator(N, km.F64, atol=cfg["atol"], c1=c1) as it:
+            it.set_omega(om)
+            it.set_factor(X, w)
+            it.build_adjacency()
+            it.set_state(y0, t0, h0)
+            it.begin(te[te > t0], 100.0, keep_rows=False, ring_rows=64)
+            st = it.advance(attempts)
+            assert st.error == 0 and st.active == 1
+            return it.get_state()
+
+    print("case fp64 N=50000 resume: 12 attempts vs 5 + restart + 7", flush=True)
+    y_full, t_full, h_full = run(th, 0.0, 0.01, 12)
+    y1, t1, h1 = run(th, 0.0, 0.01, 5)
+    y2, t2, h2 = run(y1, t1, h1, 7)
+    assert t2 == t_full and h2 == h_full
+    assert np.array_equal(y2, y_full)
