@@ -333,12 +333,12 @@ int kmt_bandwidth_probe(int device, size_t bytes, int reps, double *copy_gbps, d
   hipStream_t s = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   auto cleanup = [&]() {
-    if (e0) hipEventDestroy(e0);
-    if (e1) hipEventDestroy(e1);
-    if (a) hipFree(a);
-    if (b) hipFree(b);
-    if (sink) hipFree(sink);
-    if (s) hipStreamDestroy(s);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (sink) (void)hipFree(sink);
+    if (s) (void)hipStreamDestroy(s);
   };
   hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipMalloc(&a, n16 * 16);
@@ -351,8 +351,8 @@ int kmt_bandwidth_probe(int device, size_t bytes, int reps, double *copy_gbps, d
     cleanup();
     return fail(e == hipErrorOutOfMemory ? KMT_ERR_MEMORY : KMT_ERR_HIP, "probe setup: %s", hipGetErrorString(e));
   }
-  int cus = 256;
-  hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+  int cus = 256;  // kept if the attribute query fails
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus < 1) cus = 256;
   // The best of a few launch shapes (loads in flight per lane x workgroups
   // per CU), so that the figure is the device's ceiling, not one shape's.
   double best_copy = 0, best_read = 0;

@@ -853,18 +853,18 @@ struct kmt_batch {
 
   ~kmt_batch() { release(); }
   void drop_graphs() {
-    if (g1) hipGraphExecDestroy(g1);
-    if (g8) hipGraphExecDestroy(g8);
+    if (g1) (void)hipGraphExecDestroy(g1);
+    if (g8) (void)hipGraphExecDestroy(g8);
     g1 = g8 = nullptr;
   }
   void release() {
-    if (stream) hipStreamSynchronize(stream);
+    if (stream) (void)hipStreamSynchronize(stream);
     drop_graphs();
     void *ptrs[] = {y, ylast, ynew, om, kbuf, cs[0], cs[1], P, vthi, vtlo, err_part, ord_part, r_ring, t_eval,
                     inst, A, a_off_d, a_ld_d, items, mlp_w, log, log_count};
     for (void *p : ptrs)
-      if (p) hipFree(p);
-    if (stream) hipStreamDestroy(stream);
+      if (p) (void)hipFree(p);
+    if (stream) (void)hipStreamDestroy(stream);
     stream = nullptr;
   }
   template <class T> int alloc(T **p, size_t n) {
@@ -1010,8 +1010,8 @@ struct kmt_batch {
       e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
-    hipFree(Xd);
-    hipFree(wd);
+    (void)hipFree(Xd);
+    (void)hipFree(wd);
     if (e != hipSuccess) return kmt_fail(KMT_ERR_HIP, "adjacency build: %s", hipGetErrorString(e));
     if (shared)
       for (int q = 0; q < B; ++q) { mscale[q] = (float)sc; host_inst[q].mscale = (float)sc; }
@@ -1043,7 +1043,7 @@ struct kmt_batch {
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(out, stage, sizeof(float) * (i1 - i0) * N, hipMemcpyDeviceToHost, stream);
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
-    hipFree(stage);
+    (void)hipFree(stage);
     if (e != hipSuccess) return kmt_fail(KMT_ERR_HIP, "adjacency read: %s", hipGetErrorString(e));
     return KMT_OK;
   }
@@ -1179,10 +1179,10 @@ struct kmt_batch {
       const float *in = cs[(s - 2) & 1];
       float *out = cs[(s - 1) & 1];
       cs_cur = in;
-      if (ev) hipEventRecord((*ev)[0].first, stream);
+      if (ev) (void)hipEventRecord((*ev)[0].first, stream);
       coupling(1);
       if (ev) {
-        hipEventRecord((*ev)[0].second, stream);
+        (void)hipEventRecord((*ev)[0].second, stream);
         ev->erase(ev->begin());
       }
       epilogue(s, bvecs(in, out));
@@ -1201,7 +1201,7 @@ struct kmt_batch {
       HIP_TRY(hipStreamEndCapture(stream, &g));
       hipGraphExec_t ex = nullptr;
       hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-      hipGraphDestroy(g);
+      (void)hipGraphDestroy(g);
       if (e != hipSuccess) return kmt_fail(KMT_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
       (which ? g8 : g1) = ex;
     }
@@ -1211,7 +1211,7 @@ struct kmt_batch {
   int ensure_P() {
     const size_t need = shared ? (size_t)16 * mpad * ldn : (size_t)2 * B * ldn;
     if (P && P_elems >= need) return KMT_OK;
-    if (P) hipFree(P);
+    if (P) (void)hipFree(P);
     P = nullptr;
     KMT_TRY(alloc(&P, need));
     P_elems = need;
@@ -1237,9 +1237,9 @@ struct kmt_batch {
     drop_graphs();
     KMT_TRY(ensure_P());
     KMT_TRY(build_items());
-    if (t_eval) hipFree(t_eval);
-    if (ord_part) hipFree(ord_part);
-    if (r_ring) hipFree(r_ring);
+    if (t_eval) (void)hipFree(t_eval);
+    if (ord_part) (void)hipFree(ord_part);
+    if (r_ring) (void)hipFree(r_ring);
     t_eval = nullptr; ord_part = nullptr; r_ring = nullptr;
     n_eval = ne;
     ring_cap = rcap;
@@ -1415,11 +1415,11 @@ struct kmt_batch {
       float m = 0;
       HIP_TRY(hipEventElapsedTime(&m, p.first, p.second));
       tot += m;
-      hipEventDestroy(p.first);
-      hipEventDestroy(p.second);
+      (void)hipEventDestroy(p.first);
+      (void)hipEventDestroy(p.second);
     }
-    hipEventDestroy(r0);
-    hipEventDestroy(r1);
+    (void)hipEventDestroy(r0);
+    (void)hipEventDestroy(r1);
     if (region_ms) *region_ms = ms;
     if (coupling_ms) *coupling_ms = tot / all.size();
     KMT_TRY(fetch());
@@ -1476,8 +1476,8 @@ struct kmt_batch {
         const double score = ms / reps + partial_ms;
         if (score < best) { best = score; bnt = nt; bks = ksplit; }
       }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     tile_var = bnt;
     ksplit = bks;
     drop_graphs();
@@ -1514,8 +1514,8 @@ struct kmt_batch {
 
   int set_log(int cap) {
     drop_graphs();
-    if (log) { hipFree(log); log = nullptr; }
-    if (log_count) { hipFree(log_count); log_count = nullptr; }
+    if (log) { (void)hipFree(log); log = nullptr; }
+    if (log_count) { (void)hipFree(log_count); log_count = nullptr; }
     log_cap = std::max(cap, 0);
     if (log_cap) {
       KMT_TRY(alloc(&log, (size_t)log_cap));

@@ -100,25 +100,25 @@ template <class R> struct Engine : EngineBase {
   ~Engine() override { release(); }
 
   void release() {
-    if (stream) hipStreamSynchronize(stream);
+    if (stream) (void)hipStreamSynchronize(stream);
     drop_graphs();
     R *vecs[] = {y, ylast, ynew, omega, kbuf, cs[0], cs[1], cs_scr, vec_scr, f_scr, pairs, k_scr, X, w, XT, t_eval, ring, r_ring, times};
     for (R *p : vecs)
-      if (p) hipFree(p);
+      if (p) (void)hipFree(p);
     double *dv[] = {err_part, ord_scr, lr_part, lr_G, ord_part};
     for (double *p : dv)
-      if (p) hipFree(p);
-    if (st) hipFree(st);
-    if (st_host) hipHostFree(st_host);
-    if (A) hipFree(A);
-    if (stream) hipStreamDestroy(stream);
+      if (p) (void)hipFree(p);
+    if (st) (void)hipFree(st);
+    if (st_host) (void)hipHostFree(st_host);
+    if (A) (void)hipFree(A);
+    if (stream) (void)hipStreamDestroy(stream);
     y = ylast = ynew = omega = kbuf = nullptr;
     stream = nullptr;
   }
 
   void drop_graphs() {
-    if (g1) hipGraphExecDestroy(g1);
-    if (g8) hipGraphExecDestroy(g8);
+    if (g1) (void)hipGraphExecDestroy(g1);
+    if (g8) (void)hipGraphExecDestroy(g8);
     g1 = g8 = nullptr;
     graphs_valid = false;
   }
@@ -288,7 +288,7 @@ template <class R> struct Engine : EngineBase {
       }
       if (e == hipSuccess) e = hipStreamSynchronize(stream);
       if (e != hipSuccess) {
-        hipFree(stage);
+        (void)hipFree(stage);
         return fail(KMT_ERR_HIP, "adjacency upload failed: %s", hipGetErrorString(e));
       }
     }
@@ -322,11 +322,11 @@ template <class R> struct Engine : EngineBase {
   int set_factor(const void *Xv, const void *wv, int k_) override {
     if (k_ < 0) return fail(KMT_ERR_ARG, "rank K must be >= 0");
     drop_graphs();
-    if (X) { hipFree(X); X = nullptr; }
-    if (w) { hipFree(w); w = nullptr; }
-    if (XT) { hipFree(XT); XT = nullptr; }
-    if (lr_part) { hipFree(lr_part); lr_part = nullptr; }
-    if (lr_G) { hipFree(lr_G); lr_G = nullptr; }
+    if (X) { (void)hipFree(X); X = nullptr; }
+    if (w) { (void)hipFree(w); w = nullptr; }
+    if (XT) { (void)hipFree(XT); XT = nullptr; }
+    if (lr_part) { (void)hipFree(lr_part); lr_part = nullptr; }
+    if (lr_G) { (void)hipFree(lr_G); lr_G = nullptr; }
     K = k_;
     const long kk = std::max(K, 1);
     X_host.assign((const R *)Xv, (const R *)Xv + (size_t)N * K);
@@ -423,7 +423,7 @@ template <class R> struct Engine : EngineBase {
       if (e == hipSuccess) e = hipMemcpyAsync((R *)out + (size_t)(r0 - i0) * N, stage, sizeof(R) * nr * N, hipMemcpyDeviceToHost, stream);
       if (e == hipSuccess) e = hipStreamSynchronize(stream);
       if (e != hipSuccess) {
-        hipFree(stage);
+        (void)hipFree(stage);
         return fail(KMT_ERR_HIP, "adjacency read failed: %s", hipGetErrorString(e));
       }
     }
@@ -525,7 +525,7 @@ template <class R> struct Engine : EngineBase {
       R *out = cs[(s - 1) & 1];
       Vecs<R> v = vecs(in, out, 1);
       const int pin = (s - 2) & 1;
-      if (ev) hipEventRecord((*ev)[0].first, stream);
+      if (ev) (void)hipEventRecord((*ev)[0].first, stream);
       if (s == 2) launch_gram(in, 0, 1);
       switch (s) {
         case 2: launch_lr_stage<2, true>(v, pin); break;
@@ -536,7 +536,7 @@ template <class R> struct Engine : EngineBase {
         default: launch_lr_stage<7, false>(v, pin); break;
       }
       if (ev) {
-        hipEventRecord((*ev)[0].second, stream);
+        (void)hipEventRecord((*ev)[0].second, stream);
         ev->erase(ev->begin());
       }
     }
@@ -625,17 +625,17 @@ template <class R> struct Engine : EngineBase {
         tune_bal = bal;
         bal_grid = 0;
         launch_by_kind(3, v);
-        hipEventRecord(e0, stream);
+        HIP_TRY(hipEventRecord(e0, stream));
         for (int r = 0; r < reps; ++r) launch_by_kind(3, v);
-        hipEventRecord(e1, stream);
+        HIP_TRY(hipEventRecord(e1, stream));
         HIP_TRY(hipEventSynchronize(e1));
         float ms = 0;
         HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
         tune_ms[ri * 4 + nt * 2 + bal] = ms / reps;
         if (ms < best) { best = ms; best_rows = tune_rows; best_nt = nt; best_bal = bal; best_grid = bal_grid; }
       }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     tune_rows = best_rows;
     tune_nt = best_nt;
     tune_bal = best_bal;
@@ -669,10 +669,10 @@ template <class R> struct Engine : EngineBase {
       const R *in = cs[(s - 2) & 1];
       R *out = cs[(s - 1) & 1];
       Vecs<R> v = vecs(in, out, 1);
-      if (ev) hipEventRecord((*ev)[0].first, stream);
+      if (ev) (void)hipEventRecord((*ev)[0].first, stream);
       launch_by_kind(s, v);
       if (ev) {
-        hipEventRecord((*ev)[0].second, stream);
+        (void)hipEventRecord((*ev)[0].second, stream);
         ev->erase(ev->begin());
       }
     }
@@ -691,7 +691,7 @@ template <class R> struct Engine : EngineBase {
       HIP_TRY(hipStreamEndCapture(stream, &g));
       hipGraphExec_t ex = nullptr;
       hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-      hipGraphDestroy(g);
+      (void)hipGraphDestroy(g);
       if (e != hipSuccess) return fail(KMT_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
       if (which == 0) g1 = ex;
       else g8 = ex;
@@ -734,11 +734,11 @@ template <class R> struct Engine : EngineBase {
     if (rcap < 1) return fail(KMT_ERR_ARG, "ring_rows must be >= 1");
     HIP_TRY(hipStreamSynchronize(stream));
     drop_graphs();
-    if (t_eval) { hipFree(t_eval); t_eval = nullptr; }
-    if (ring) { hipFree(ring); ring = nullptr; }
-    if (r_ring) { hipFree(r_ring); r_ring = nullptr; }
-    if (times) { hipFree(times); times = nullptr; }
-    if (ord_part) { hipFree(ord_part); ord_part = nullptr; }
+    if (t_eval) { (void)hipFree(t_eval); t_eval = nullptr; }
+    if (ring) { (void)hipFree(ring); ring = nullptr; }
+    if (r_ring) { (void)hipFree(r_ring); r_ring = nullptr; }
+    if (times) { (void)hipFree(times); times = nullptr; }
+    if (ord_part) { (void)hipFree(ord_part); ord_part = nullptr; }
     n_eval = ne;
     keep_rows = keep;
     ring_cap = rcap;
@@ -905,8 +905,8 @@ template <class R> struct Engine : EngineBase {
     HIP_TRY(hipEventSynchronize(e1));
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     if (avg_ms) *avg_ms = ms / reps;
     return KMT_OK;
   }
@@ -938,11 +938,11 @@ template <class R> struct Engine : EngineBase {
       float m = 0;
       HIP_TRY(hipEventElapsedTime(&m, p.first, p.second));
       tot += m;
-      hipEventDestroy(p.first);
-      hipEventDestroy(p.second);
+      (void)hipEventDestroy(p.first);
+      (void)hipEventDestroy(p.second);
     }
-    hipEventDestroy(r0);
-    hipEventDestroy(r1);
+    (void)hipEventDestroy(r0);
+    (void)hipEventDestroy(r1);
     if (region_ms) *region_ms = ms;
     if (stage_avg_ms) *stage_avg_ms = tot / pairs_all.size();
     KMT_TRY(fetch_state());

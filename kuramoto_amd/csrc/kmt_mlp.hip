@@ -142,11 +142,11 @@ struct kmt_mlp {
   long step = 0;
 
   ~kmt_mlp() {
-    if (stream) hipStreamSynchronize(stream);
+    if (stream) (void)hipStreamSynchronize(stream);
     void *p[] = {w32, m, v, grad, w16, X, y, H1, H2, D1, D2, D3, out, lpart};
     for (void *q : p)
-      if (q) hipFree(q);
-    if (stream) hipStreamDestroy(stream);
+      if (q) (void)hipFree(q);
+    if (stream) (void)hipStreamDestroy(stream);
   }
 
   int init(int dev, const float *w_init, uint32_t seed) {
@@ -185,7 +185,7 @@ struct kmt_mlp {
     if (n <= cap) return KMT_OK;
     void *p[] = {X, y, H1, H2, D1, D2, D3, out, lpart};
     for (void *q : p)
-      if (q) hipFree(q);
+      if (q) (void)hipFree(q);
     cap = n;
     HIP_TRY(hipMalloc((void **)&X, sizeof(float) * kIn * n));
     HIP_TRY(hipMalloc((void **)&y, sizeof(float) * n));
@@ -216,14 +216,16 @@ struct kmt_mlp {
     return KMT_OK;
   }
 
-  double loss_sum(int S) {
+  // sum of the per-block loss partials of the last fwd_bwd over S samples
+  int loss_sum(int S, double *out) {
     const int blocks = (S + 3) / 4;
     std::vector<double> lp(blocks);
-    hipMemcpyAsync(lp.data(), lpart, sizeof(double) * blocks, hipMemcpyDeviceToHost, stream);
-    hipStreamSynchronize(stream);
+    HIP_TRY(hipMemcpyAsync(lp.data(), lpart, sizeof(double) * blocks, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
     double t = 0;
     for (double q : lp) t += q;
-    return t;
+    *out = t;
+    return KMT_OK;
   }
 };
 
@@ -275,7 +277,11 @@ int kmt_mlp_loss_grad(kmt_mlp *m, const float *Xh, const float *yh, int n, doubl
   HIP_TRY(hipGetLastError());
   if (gradh) HIP_TRY(hipMemcpyAsync(gradh, m->grad, sizeof(float) * kNParams, hipMemcpyDeviceToHost, m->stream));
   HIP_TRY(hipStreamSynchronize(m->stream));
-  if (loss) *loss = m->loss_sum(n) / n;
+  if (loss) {
+    double t = 0;
+    KMT_TRY(m->loss_sum(n, &t));
+    *loss = t / n;
+  }
   return KMT_OK;
 }
 
@@ -294,7 +300,9 @@ int kmt_mlp_train(kmt_mlp *m, const float *Xh, const float *yh, int n, int epoch
       mlp_adam_kernel<<<(kNParams + kBlock - 1) / kBlock, kBlock, 0, m->stream>>>(m->w32, m->w16, m->m, m->v, m->grad,
                                                                                    (float)lr, b1, b2, eps, c1, c2);
       HIP_TRY(hipGetLastError());
-      tot += m->loss_sum(S);
+      double part = 0;
+      KMT_TRY(m->loss_sum(S, &part));
+      tot += part;
     }
     if (loss_history) loss_history[ep] = tot / n;
   }
