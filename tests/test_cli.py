@@ -224,3 +224,23 @@ def test_dmd_load_data_on_oracle_outputs(tmp_path, orc):
     om = np.fromfile(f"{o0}0frequencies.dat", dtype=np.float64)
     assert raw.shape[0] == 9 and np.array_equal(theta[0], raw[0])
     assert np.allclose(theta[3], raw[3] - om.mean() * 0.125 * 3, rtol=0, atol=1e-12)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prog_name,dtype_opt", [("kuramoto", "bf16"), ("kuramoto", "f16"), ("kuramoto_64", "bf16")])
+def test_cli_16bit_adjacency(tmp_path, km, prog_name, dtype_opt):
+    """`--adj-dtype=bf16|f16` (A stored in 16 bits, FP32 accumulation; with
+    kuramoto_64 the state, stage combinations and error norm stay FP64)
+    writes the same files as the dense run and follows it: order.dat within
+    1e-3, the same number of output rows, accepted steps within 2% (or 1)."""
+    prog = os.path.join(BIN, prog_name)
+    dt = np.float32 if prog_name == "kuramoto" else np.float64
+    q, d = str(tmp_path / "q"), str(tmp_path / "d")
+    args = ARGS + ["-D", "2", "-a", "1e-6" if prog_name == "kuramoto" else "1e-9"]
+    print(f"case cli {prog_name} --adj-dtype={dtype_opt}", flush=True)
+    run(prog, args + [f"--adj-dtype={dtype_opt}"], q)
+    run(prog, args, d)
+    check_layout(q, 120, N_OUT, dt, dense=2)
+    assert np.max(np.abs(load(q, "order.dat", dt) - load(d, "order.dat", dt))) < 1e-3
+    nq, nd = load(q, "times.dat", dt).size, load(d, "times.dat", dt).size
+    assert abs(nq - nd) <= max(1, int(0.02 * nd)), (nq, nd)
