@@ -244,3 +244,34 @@ def test_cli_16bit_adjacency(tmp_path, km, prog_name, dtype_opt):
     assert np.max(np.abs(load(q, "order.dat", dt) - load(d, "order.dat", dt))) < 1e-3
     nq, nd = load(q, "times.dat", dt).size, load(d, "times.dat", dt).size
     assert abs(nq - nd) <= max(1, int(0.02 * nd)), (nq, nd)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("extra", [["-F"], ["-f", "0.5"], ["-r", "1e-6"], ["--gauss"], ["-v"]],
+                         ids=["fixed", "freq", "rtol", "gauss", "verbose"])
+def test_cli_options_match_oracle_cli(tmp_path, km, orc, extra):
+    """The remaining reference options (-F fixed step, -f frequency width,
+    -r relative tolerance, -v verbose; kuramoto.cu:298-362) and --gauss on
+    kuramoto_64 against the oracle's program with the same arguments: same
+    frequencies, order.dat and phases within the FP64 tolerances, accepted
+    steps within 2% (or 1).  --chunk (how many attempts run between host
+    reads) must not change any file."""
+    prog = os.path.join(BIN, "kuramoto_64")
+    oprog = os.path.join(ORC, "kuramoto_oracle_64")
+    dt = np.float64
+    g, o, c = str(tmp_path / "g"), str(tmp_path / "o"), str(tmp_path / "c")
+    args = ARGS + ["-D", "2", "-a", "1e-9"] + extra
+    print(f"case cli option {' '.join(extra)}", flush=True)
+    run(prog, args, g)
+    run(oprog, args, o)
+    run(prog, args + ["--chunk=3"], c)
+    check_layout(g, 120, N_OUT, dt, dense=2)
+    assert open(g + "frequencies.dat", "rb").read() == open(o + "frequencies.dat", "rb").read()
+    assert np.max(np.abs(load(g, "order.dat", dt) - load(o, "order.dat", dt))) < 1e-9
+    tg, to = load(g, "times.dat", dt), load(o, "times.dat", dt)
+    assert abs(len(tg) - len(to)) <= max(1, int(0.02 * len(to)))
+    thg = load(g, "thetas.dat", dt).reshape(N_OUT, 120)
+    tho = load(o, "thetas.dat", dt).reshape(N_OUT, 120)
+    assert np.max(np.abs(thg - tho) / np.maximum(1, np.abs(tho))) < 1e-9
+    for suf in ("order.dat", "thetas.dat", "couplings.dat", "times.dat", "fs.dat"):
+        assert open(g + suf, "rb").read() == open(c + suf, "rb").read(), suf
