@@ -275,3 +275,24 @@ def test_cli_options_match_oracle_cli(tmp_path, km, orc, extra):
     assert np.max(np.abs(thg - tho) / np.maximum(1, np.abs(tho))) < 1e-9
     for suf in ("order.dat", "thetas.dat", "couplings.dat", "times.dat", "fs.dat"):
         assert open(g + suf, "rb").read() == open(c + suf, "rb").read(), suf
+
+
+@pytest.mark.parametrize("args", [[], ["-h"], ["-N", "64", "-h"], ["BASE", "-h"]],
+                         ids=["no-filebase", "h", "h-after-option", "h-after-filebase"])
+@pytest.mark.parametrize("prog", ["kuramoto", "kuramoto_64"])
+def test_cli_usage_exits_without_running(tmp_path, orc, prog, args):
+    """kuramoto.cu:298-388: with -h, or without a FILEBASE, or with -h after
+    the FILEBASE, main prints the usage text and exits 0 before any file or
+    device is touched.  Runs on the CPU: no HIP call is made.  The text
+    equals the oracle program's line for line; each adds one last line
+    naming its own long-option extensions."""
+    args = [str(tmp_path / "run") if a == "BASE" else a for a in args]
+    got = subprocess.run([os.path.join(BIN, prog)] + args, capture_output=True, text=True, timeout=60)
+    ref = subprocess.run([os.path.join(ORC, "kuramoto_oracle")] + args, capture_output=True, text=True, timeout=60)
+    assert got.returncode == 0 and ref.returncode == 0
+    lines, ref_lines = got.stdout.splitlines(), ref.stdout.splitlines()
+    assert lines[0] == "usage:\tkuramoto [-hvnRFA] [-N N] [-K K] [-D D]"
+    assert lines[-2] == ref_lines[-2] == "filebase is base file name for output. "
+    assert lines[:-1] == ref_lines[:-1] and len(lines) == 24
+    assert lines[-1] == "extensions: --gauss --adj-dtype=native|bf16|f16 --rebuild --max-steps=n --chunk=n"
+    assert os.listdir(tmp_path) == []
