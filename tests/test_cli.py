@@ -296,3 +296,23 @@ def test_cli_usage_exits_without_running(tmp_path, orc, prog, args):
     assert lines[:-1] == ref_lines[:-1] and len(lines) == 24
     assert lines[-1] == "extensions: --gauss --adj-dtype=native|bf16|f16 --rebuild --max-steps=n --chunk=n"
     assert os.listdir(tmp_path) == []
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prog,N", [("kuramoto", 300000), ("kuramoto_64", 200000)])
+def test_cli_memory_check_stops_too_large_run(tmp_path, km, prog, N):
+    """kuramoto.cu:424-438: when the device's free memory is below the
+    estimate (here the N^2 adjacency alone, 360 / 320 GB, exceeds the 288 GB
+    of HBM), main writes the header, the command line and the two memory
+    lines to FILEBASE.out and returns 0 before allocating anything."""
+    base = str(tmp_path / "big")
+    print(f"case {prog} N={N} memory check", flush=True)
+    out = run(os.path.join(BIN, prog), ["-N", str(N), "-K", "5", "-D", "0"], base, timeout=120)
+    assert "GPU Memory low!" in out.splitlines()
+    lines = open(base + ".out").read().splitlines()
+    assert lines[0].split()[0] == str(N) and lines[-1] == "GPU Memory low!"
+    fr, total, req = (int(v) for v in lines[-2].split()[2:])
+    assert lines[-2].startswith("GPU Memory: ") and fr < req and total < req
+    s = 4 if prog == "kuramoto" else 8
+    assert req == (100 + 2 * 5) * N * s + N * N * s
+    assert sorted(os.listdir(tmp_path)) == ["big.out"]
