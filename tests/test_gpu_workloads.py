@@ -16,6 +16,9 @@
 * The shared-adjacency batch at its full size (B = 64, N = 20000, BF16)
   against plain FP32 runs of three of its instances on the same stored
   matrix, with the row compaction exercised by the end of the run.
+* The own-adjacency batch at its full size (B = 48, N_b in 5000..40000,
+  64.6 GB of BF16 matrices): four instances equal single BF16 runs bit for
+  bit.
 """
 import os
 
@@ -139,3 +142,48 @@ def test_shared_batch64_n20000_matches_single_runs(km):
         tot_b += finals[b]["accepted"]
         tot_s += acc
     assert abs(tot_b - tot_s) <= max(1, int(0.02 * tot_s)), (tot_b, tot_s)
+
+
+def test_uneven_batch48_equals_single_runs_bit_for_bit(km):
+    """The own-adjacency batch at its full size (B = 48, N_b uniform in
+    {5000, ..., 40000}, each with its own BF16 matrix from seeds 100..147,
+    normal and Cauchy frequencies alternating, c = 3).  Grouped mode runs
+    each instance through the single run's gemv-pair loop (8-row groups),
+    epilogue and controller, and sums its error partials in the same order.
+    So the smallest, the largest and two more instances equal single BF16
+    runs with 8-row groups, bit for bit, to t = 1: r(t), final phases, t,
+    and accepted and rejected counts.  At these sizes each instance has
+    625..5000 error partials, more than one per thread of the controller's
+    sum."""
+    from kuramoto_amd.workloads import BATCH, make_batch
+
+    cfg = BATCH["batch48-uneven-bf16"]
+    t1 = 1.0
+    te = _grid(t1, 0.25).astype(np.float64)
+    print("case batch48-uneven-bf16 full size vs single runs", flush=True)
+    bt, Ns, _ = make_batch("batch48-uneven-bf16", proposer=False)
+    picks = sorted({int(np.argmin(Ns)), int(np.argmax(Ns)), 23, 46})
+    try:
+        res = bt.run(te, t1, ring_rows=64, chunk=8)
+        finals = {b: bt.get_instance(b) for b in picks}
+    finally:
+        bt.close()
+    assert res["status"].active == 0 and res["status"].errors == 0
+    for b in picks:
+        N, seed = Ns[b], cfg["seeds"][b]
+        th, om = km.draws(km.F32, N, seed, normal=cfg["normal"][b], init=6.283185307)
+        X = km.factor(N, 5, seed, gaussian=True).astype(np.float32)
+        w = km.factor_weights(km.F32, N, 5, gaussian=True)
+        with km.Integrator(N, km.F32, atol=1e-6, c1=float(cfg["cs"][b])) as it:
+            it.set_omega(om)
+            it.set_factor(X, w)
+            it.build_adjacency("bf16")
+            it.select_variant(8, nontemporal=True, balanced=False)
+            it.set_state(th, 0.0, 0.01)
+            single = it.run(te, t1, keep_rows=False, ring_rows=64)
+            y, t, h = it.get_state()
+        assert np.array_equal(res["r"][b], single["r"]), (b, N)
+        assert finals[b][1]["accepted"] == single["status"].accepted, (b, N)
+        assert finals[b][1]["rejected"] == single["status"].rejected, (b, N)
+        assert np.array_equal(finals[b][0], y), (b, N)
+        assert np.float32(finals[b][1]["t"]) == np.float32(t), (b, N)
