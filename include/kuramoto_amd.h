@@ -43,6 +43,9 @@ extern "C" {
 #define KMT_ADJ_NATIVE 0 /* same type as the working precision */
 #define KMT_ADJ_BF16 2   /* bfloat16 with FP32 accumulation (KMT_F32 only) */
 #define KMT_ADJ_F16 3    /* IEEE half with FP32 accumulation (KMT_F32 only) */
+#define KMT_ADJ_SYM 4    /* working precision, upper block triangle of a symmetric
+                            matrix (half the bytes per RHS); no reference
+                            counterpart: its gemvs read the full matrix */
 
 typedef struct kmt_ctx kmt_ctx;
 
@@ -111,7 +114,8 @@ int kmt_get_state(kmt_ctx *ctx, void *theta, double *t, double *h);
 /* Dense adjacency from host memory (the -R adj.dat path, kuramoto.cu:559-566).
  * M: N x N values of type `src_precision` with row stride ld (>= N).
  * storage: KMT_ADJ_NATIVE / _BF16 / _F16 (reduced storage uses a power-of-two
- * scale chosen from max|M| so the stored values are O(1)). */
+ * scale chosen from max|M| so the stored values are O(1)), or KMT_ADJ_SYM
+ * (M must equal its transpose exactly; KMT_ERR_ARG otherwise). */
 int kmt_set_adjacency(kmt_ctx *ctx, const void *M, int src_precision, long ld, int storage);
 /* Low-rank factor for the adjacency: X (N x K, row-major, context
  * precision) and weights w (K).  Used by kmt_build_adjacency and -A. */

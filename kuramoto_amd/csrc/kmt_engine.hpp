@@ -16,6 +16,7 @@
 #include "../../include/kuramoto_amd.h"
 #include "kmt_common.hpp"
 #include "kmt_kernels.hpp"
+#include "kmt_sym.hpp"
 
 namespace kmt {
 
@@ -79,6 +80,10 @@ template <class R> struct Engine : EngineBase {
   R *XT = nullptr;  // transposed factor [K][ldv] for the register-rebuild form
   int K = 0;
   int factored = 0;  // 0 dense A, KMT_LOWRANK_CONTRACT, KMT_LOWRANK_REBUILD
+  // KMT_ADJ_SYM: (I, J) of each stored block, and the row / column partials
+  long sym_nb = 0, sym_blocks = 0;
+  int2 *sym_ij = nullptr;
+  SymPair<R> *sym_prow = nullptr, *sym_pcol = nullptr;
   double *lr_part = nullptr, *lr_G = nullptr;
   int lr_nblk = 0, lr_grid = 0;
   std::vector<R> X_host, w_host;
@@ -109,6 +114,9 @@ template <class R> struct Engine : EngineBase {
     for (double *p : dv)
       if (p) (void)hipFree(p);
     if (st) (void)hipFree(st);
+    if (sym_ij) (void)hipFree(sym_ij);
+    if (sym_prow) (void)hipFree(sym_prow);
+    if (sym_pcol) (void)hipFree(sym_pcol);
     if (st_host) (void)hipHostFree(st_host);
     if (A) (void)hipFree(A);
     if (stream) (void)hipStreamDestroy(stream);
@@ -171,7 +179,7 @@ template <class R> struct Engine : EngineBase {
   }
 
   int layout(long *plda, int *eb, int *rows, int *flags) override {
-    if (plda) *plda = lda;
+    if (plda) *plda = storage == KMT_ADJ_SYM ? kSymT : lda;
     if (eb) *eb = storage == KMT_ADJ_BF16 || storage == KMT_ADJ_F16 ? 2 : (int)sizeof(R);
     if (rows) *rows = tune_rows;
     if (flags) *flags = (tune_nt ? 1 : 0) | (tune_bal ? 2 : 0);
@@ -243,10 +251,30 @@ template <class R> struct Engine : EngineBase {
   // ---------------- adjacency ----------------
   size_t elem_bytes(int stg) const { return (stg == KMT_ADJ_BF16 || stg == KMT_ADJ_F16) ? 2 : sizeof(R); }
 
+  // block layout of KMT_ADJ_SYM: the (I, J) table and the partial buffers
+  int ensure_sym() {
+    if (sym_ij) return KMT_OK;
+    sym_nb = (N + kSymT - 1) / kSymT;
+    sym_blocks = sym_nb * (sym_nb + 1) / 2;
+    std::vector<int2> ij((size_t)sym_blocks);
+    for (long I = 0; I < sym_nb; ++I)
+      for (long J = I; J < sym_nb; ++J) ij[(size_t)sym_index(I, J, sym_nb)] = make_int2((int)I, (int)J);
+    HIP_TRY(hipMalloc((void **)&sym_ij, sizeof(int2) * ij.size()));
+    HIP_TRY(hipMemcpyAsync(sym_ij, ij.data(), sizeof(int2) * ij.size(), hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipMalloc((void **)&sym_prow, sizeof(SymPair<R>) * (size_t)sym_blocks * kSymT));
+    HIP_TRY(hipMalloc((void **)&sym_pcol, sizeof(SymPair<R>) * (size_t)sym_blocks * kSymT));
+    HIP_TRY(hipMemsetAsync(sym_prow, 0, sizeof(SymPair<R>) * (size_t)sym_blocks * kSymT, stream));
+    HIP_TRY(hipMemsetAsync(sym_pcol, 0, sizeof(SymPair<R>) * (size_t)sym_blocks * kSymT, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    return KMT_OK;
+  }
+
   int ensure_A(int stg) {
-    if (stg != KMT_ADJ_NATIVE && stg != KMT_ADJ_BF16 && stg != KMT_ADJ_F16)
+    if (stg != KMT_ADJ_NATIVE && stg != KMT_ADJ_BF16 && stg != KMT_ADJ_F16 && stg != KMT_ADJ_SYM)
       return fail(KMT_ERR_ARG, "unknown adjacency storage %d", stg);
-    const size_t bytes = (size_t)rows_alloc * (size_t)lda * elem_bytes(stg);
+    if (stg == KMT_ADJ_SYM) KMT_TRY(ensure_sym());
+    const size_t bytes = stg == KMT_ADJ_SYM ? (size_t)sym_blocks * kSymT * kSymT * sizeof(R)
+                                            : (size_t)rows_alloc * (size_t)lda * elem_bytes(stg);
     if (A && A_bytes == bytes && storage == stg) return KMT_OK;
     drop_graphs();
     if (A) {
@@ -300,8 +328,19 @@ template <class R> struct Engine : EngineBase {
     if (ld < N) return fail(KMT_ERR_ARG, "adjacency row stride %ld < N=%ld", ld, N);
     if (src_prec != (sizeof(R) == 4 ? KMT_F32 : KMT_F64))
       return fail(KMT_ERR_ARG, "adjacency source precision must match the context precision");
-    KMT_TRY(ensure_A(stg));
     const R *M = (const R *)Mv;
+    if (stg == KMT_ADJ_SYM) {
+      for (long i = 0; i < N; ++i)
+        for (long j = i + 1; j < N; ++j)
+          if (!(M[(size_t)i * ld + j] == M[(size_t)j * ld + i]) &&
+              !(std::isnan((double)M[(size_t)i * ld + j]) && std::isnan((double)M[(size_t)j * ld + i])))
+            return fail(KMT_ERR_ARG, "KMT_ADJ_SYM needs a symmetric matrix: M[%ld][%ld] != M[%ld][%ld]", i, j, j, i);
+      KMT_TRY(ensure_A(stg));
+      mscale_ = 1.0;
+      factored = 0;
+      return upload_sym(M, ld);
+    }
+    KMT_TRY(ensure_A(stg));
     if (stg == KMT_ADJ_NATIVE) {
       mscale_ = 1.0;
       HIP_TRY(hipMemcpy2DAsync(A, sizeof(R) * lda, M, sizeof(R) * ld, sizeof(R) * N, N, hipMemcpyHostToDevice, stream));
@@ -317,6 +356,30 @@ template <class R> struct Engine : EngineBase {
     const R inv = (R)(1.0 / mscale_);
     if (stg == KMT_ADJ_BF16) return upload_rows<bf16_t>(M, ld, inv);
     return upload_rows<f16_t>(M, ld, inv);
+  }
+
+  // pack the upper block triangle of a host matrix, a few blocks per copy
+  int upload_sym(const R *M, long ld) {
+    const long per = 32;
+    std::vector<R> buf((size_t)per * kSymT * kSymT);
+    for (long b0 = 0; b0 < sym_blocks; b0 += per) {
+      const long nbk = std::min(per, sym_blocks - b0);
+      std::fill(buf.begin(), buf.end(), (R)0);
+      long b = 0;
+      for (long I = 0; I < sym_nb && b < b0 + nbk; ++I)
+        for (long J = I; J < sym_nb; ++J, ++b) {
+          if (b < b0) continue;
+          if (b >= b0 + nbk) break;
+          R *dst = buf.data() + (size_t)(b - b0) * kSymT * kSymT;
+          for (long r = 0; r < kSymT && I * kSymT + r < N; ++r)
+            for (long c = 0; c < kSymT && J * kSymT + c < N; ++c)
+              dst[(size_t)r * kSymT + c] = M[(size_t)(I * kSymT + r) * ld + J * kSymT + c];
+        }
+      HIP_TRY(hipMemcpyAsync((R *)A + (size_t)b0 * kSymT * kSymT, buf.data(), sizeof(R) * (size_t)nbk * kSymT * kSymT,
+                             hipMemcpyHostToDevice, stream));
+      HIP_TRY(hipStreamSynchronize(stream));  // buf is refilled next
+    }
+    return KMT_OK;
   }
 
   int set_factor(const void *Xv, const void *wv, int k_) override {
@@ -371,6 +434,13 @@ template <class R> struct Engine : EngineBase {
       mscale_ = 1.0;
       return gen_rows<R>((R)1);
     }
+    if (stg == KMT_ADJ_SYM) {
+      mscale_ = 1.0;
+      gen_adj_sym_kernel<R><<<(unsigned)sym_blocks, kBlock, 0, stream>>>(X, w, K, N, sym_ij, (R *)A);
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipStreamSynchronize(stream));
+      return KMT_OK;
+    }
     // |M_ij| <= max_n sum_m |w_m| X_nm^2 (Cauchy–Schwarz): scale so |stored| <= 1
     double mx = 0;
     for (long n = 0; n < N; ++n) {
@@ -413,7 +483,9 @@ template <class R> struct Engine : EngineBase {
       const long nr = std::min(chunk, i1 - r0);
       dim3 grid((unsigned)std::min<long>((N + kBlock - 1) / kBlock, 64), (unsigned)nr);
       hipError_t e = hipSuccess;
-      if (storage == KMT_ADJ_NATIVE)
+      if (storage == KMT_ADJ_SYM)
+        read_sym_rows_kernel<R><<<grid, kBlock, 0, stream>>>((const R *)A, sym_nb, N, r0, nr, stage);
+      else if (storage == KMT_ADJ_NATIVE)
         read_rows_kernel<R, R><<<grid, kBlock, 0, stream>>>((const R *)A + (size_t)r0 * lda, lda, N, nr, (R)1, stage);
       else if (storage == KMT_ADJ_BF16)
         read_rows_kernel<R, bf16_t><<<grid, kBlock, 0, stream>>>((const bf16_t *)A + (size_t)r0 * lda, lda, N, nr, (R)mscale_, stage);
@@ -489,8 +561,22 @@ template <class R> struct Engine : EngineBase {
     }
   }
 
+  // KMT_ADJ_SYM: block kernel over the stored triangle, then the row reduction
+  template <int KIND> void launch_sym(const Vecs<R> &v) {
+    SymArgs<R> a;
+    a.A = (const R *)A;
+    a.ij = sym_ij;
+    a.nb = sym_nb;
+    a.prow = sym_prow;
+    a.pcol = sym_pcol;
+    a.v = v;
+    sym_tile_kernel<R><<<(unsigned)sym_blocks, kBlock, 0, stream>>>(a);
+    sym_reduce_kernel<R, KIND><<<(unsigned)sym_nb, kBlock, 0, stream>>>(a);
+  }
+
   template <int KIND> void launch_dense(const Vecs<R> &v) {
-    if (storage == KMT_ADJ_NATIVE) launch_dense_kind<R, KIND>(v);
+    if (storage == KMT_ADJ_SYM) launch_sym<KIND>(v);
+    else if (storage == KMT_ADJ_NATIVE) launch_dense_kind<R, KIND>(v);
     else if (storage == KMT_ADJ_BF16) launch_dense_kind<bf16_t, KIND>(v);
     else launch_dense_kind<f16_t, KIND>(v);
   }
@@ -593,6 +679,7 @@ template <class R> struct Engine : EngineBase {
   int n_parts_stage7() const {
     if (factored == KMT_LOWRANK_REBUILD) return (int)((N + rebuild_rows() - 1) / rebuild_rows());
     if (factored) return lr_fused() ? lr_grid : (int)((N + kBlock - 1) / kBlock);
+    if (storage == KMT_ADJ_SYM) return (int)sym_nb;
     if (tune_bal) return bal_grid;
     return (int)((N + tune_rows - 1) / tune_rows);
   }
@@ -604,7 +691,7 @@ template <class R> struct Engine : EngineBase {
   // Every variant sums each row in the same order, so the choice changes
   // speed only, never results.
   int autotune(int reps) {
-    if (factored || !A) return KMT_OK;
+    if (factored || !A || storage == KMT_ADJ_SYM) return KMT_OK;  // one form of the symmetric kernels
     if (!k_scr) {
       HIP_TRY(hipMalloc((void **)&k_scr, sizeof(R) * 7 * ldv));
       HIP_TRY(hipMemsetAsync(k_scr, 0, sizeof(R) * 7 * ldv, stream));

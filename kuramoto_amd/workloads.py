@@ -24,6 +24,12 @@ PLAIN = {
     # FP32 dense at the same size (10 GB adjacency)
     "fp32-n50000": dict(N=50000, K=5, seed=2, prec="f32", normal=True, init=6.283185307, c1=3.0, atol=1e-6,
                         t1=100.0, dt=1.0, storage="native"),
+    # the same two runs with the adjacency stored as its upper block triangle
+    # (symmetric storage: 10 GB / 5 GB, half the bytes per RHS)
+    "fp64-n50000-sym": dict(N=50000, K=5, seed=2, prec="f64", normal=True, init=6.283185307, c1=3.0, atol=1e-9,
+                            t1=100.0, dt=1.0, storage="sym"),
+    "fp32-n50000-sym": dict(N=50000, K=5, seed=2, prec="f32", normal=True, init=6.283185307, c1=3.0, atol=1e-6,
+                            t1=100.0, dt=1.0, storage="sym"),
     # the reference's default FP32 run: N=5000, Cauchy, theta0=0 (A fits the 256 MB cache)
     "fp32-n5000": dict(N=5000, K=5, seed=1, prec="f32", normal=False, init=0.0, c1=3.0, atol=1e-6, t1=100.0,
                        dt=1.0, storage="native"),

@@ -1,0 +1,161 @@
+"""GPU parity of the symmetric-storage path (KMT_ADJ_SYM): the upper block
+triangle of a symmetric adjacency in 256 x 256 blocks, one block kernel
+(row sums, and column sums standing in for the lower triangle) and one row
+reduction + stage epilogue per RHS.  The reference reads the full matrix in
+its two gemvs (kuramoto.cu:104-126); only the summation order differs.
+
+* The stored triangle, read back as full rows, equals the dense adjacency
+  of the same factor bit for bit (the generator's formula on both sides).
+  N on both sides of the block edge.
+* A symmetric host matrix packed by kmt_set_adjacency reads back unchanged;
+  a non-symmetric one is refused (KMT_ERR_ARG).
+* RHS and couplings against the oracle, with the dense path's tolerances
+  (FP64 1e-12, FP32 2e-6 of the row scale sum_j |M_ij|).
+* Whole integrations against the oracle, with the dense path's tolerances.
+  Two runs agree bit for bit.
+* At N = 50000 in FP64 (10 GB stored instead of 20 GB), the RHS equals the
+  dense path's to 1e-12 of the row scale.  The symmetric-M invariant
+  sum_i f_i = sum_i w_i holds, and so does the drift invariant through
+  whole steps.
+"""
+import numpy as np
+import pytest
+
+from kuramoto_amd.workloads import PLAIN
+
+pytestmark = pytest.mark.gpu
+
+
+def _grid(t1, dt, dtype):
+    dt = dtype(dt)
+    return (dt * np.arange(1, int(dtype(t1) / dt) + 1)).astype(dtype)
+
+
+def _factor_case(km, prec, N, seed, K=5):
+    dtype = np.float32 if prec == km.F32 else np.float64
+    theta0, omega = km.draws(prec, N, seed, normal=True, init=6.283185307)
+    X = km.factor(N, K, seed, gaussian=True).astype(dtype)
+    w = km.factor_weights(prec, N, K, gaussian=True)
+    return dtype, theta0, omega, X, w
+
+
+@pytest.mark.parametrize("N", [1, 255, 256, 257, 600])
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_sym_storage_equals_dense(km, prec, N):
+    p = km.F32 if prec == "f32" else km.F64
+    dtype, theta0, omega, X, w = _factor_case(km, p, N, 7)
+    print(f"case sym storage N={N} {prec}", flush=True)
+    with km.Integrator(N, p, c1=3.0) as it:
+        it.set_factor(X, w)
+        it.build_adjacency("native")
+        dense = it.adjacency_rows(0, N)
+        it.build_adjacency("sym")
+        packed = it.adjacency_rows(0, N)
+    assert np.array_equal(dense, packed)
+    assert np.array_equal(packed, packed.T)
+
+
+def test_sym_upload_packs_and_checks_symmetry(km):
+    N = 300
+    rng = np.random.default_rng(5)
+    M = rng.standard_normal((N, N))
+    M = M + M.T
+    print("case sym upload", flush=True)
+    with km.Integrator(N, km.F64, c1=1.0) as it:
+        it.set_adjacency(M, "sym")
+        assert np.array_equal(it.adjacency_rows(0, N), M)
+        M[3, 200] += 1e-9
+        with pytest.raises(km.KmtError, match="symmetric"):
+            it.set_adjacency(M, "sym")
+
+
+@pytest.mark.parametrize("N", [1, 2, 65, 255, 257, 1000, 4097])
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_sym_rhs_matches_oracle(km, orc, N, prec):
+    dtype = np.float32 if prec == "f32" else np.float64
+    p = km.F32 if prec == "f32" else km.F64
+    rng = np.random.default_rng(11 + N)
+    M = rng.standard_normal((N, N)) / max(N, 1)
+    M = (0.5 * (M + M.T)).astype(dtype)
+    omega = rng.standard_normal(N).astype(dtype)
+    theta = (rng.uniform(-np.pi, np.pi, N) * 3).astype(dtype)
+    c1 = 1.75
+    print(f"case sym rhs N={N} {prec}", flush=True)
+    with km.Integrator(N, p, c1=c1) as it:
+        it.set_omega(omega)
+        it.set_adjacency(M, "sym")
+        f, cp = it.rhs(theta, couplings=True)
+    fo, cso = orc.rhs(orc.Adj(0, M=M), omega, c1, theta, dtype=dtype)
+    rs = np.abs(M.astype(np.float64)).sum(axis=1)
+    tol = (2e-6 if prec == "f32" else 1e-12) * (np.abs(omega.astype(np.float64)) + c1 * rs)
+    assert np.all(np.abs(f.astype(np.float64) - fo) <= tol + 1e-30)
+    tolc = (2e-6 if prec == "f32" else 1e-12) * np.repeat(rs, 2)
+    assert np.all(np.abs(cp.astype(np.float64) - cso) <= tolc + 1e-30)
+
+
+@pytest.mark.parametrize("prec,atol", [("f64", 1e-9), ("f32", 1e-6)])
+@pytest.mark.parametrize("N", [333, 2000])
+def test_sym_integration_matches_oracle(km, orc, prec, atol, N):
+    """Same acceptance as the dense path: accepted steps within 2% (or 1),
+    r(t) within 1e-3; FP64 phases at t=1 within 1e-10 of the FP64 oracle;
+    FP32 phases no further from an FP64 truth run (atol 1e-12) than twice
+    the FP32 oracle's distance + 1e-5."""
+    p = km.F32 if prec == "f32" else km.F64
+    c1, t1 = 3.0, 1.0
+    dtype, theta0, omega, X, w = _factor_case(km, p, N, 2)
+    te = _grid(t1, 0.1, dtype)
+    print(f"case sym integration N={N} {prec}", flush=True)
+    runs = []
+    for _ in range(2):
+        with km.Integrator(N, p, atol=atol, c1=c1) as it:
+            it.set_omega(omega)
+            it.set_factor(X, w)
+            it.build_adjacency("sym")
+            it.set_state(theta0, 0.0, 0.01)
+            runs.append(it.run(te, t1, keep_rows=True, ring_rows=8))
+    g, g2 = runs
+    assert np.array_equal(g["rows"], g2["rows"]) and np.array_equal(g["r"], g2["r"])
+    M = orc.adj_rows(X, w, 0, N)
+    ref = orc.integrate(orc.Adj(0, M=M), omega, c1, theta0, 0.0, 0.01, t1, atol, t_eval=te, dtype=dtype)
+    st = g["status"]
+    assert abs(st.accepted - ref["accepted"]) <= max(1, int(0.02 * ref["accepted"]))
+    assert np.max(np.abs(g["r"] - ref["r"])) < 1e-3
+    if prec == "f64":
+        rel = np.abs(g["rows"][-1] - ref["rows"][-1]) / np.maximum(1.0, np.abs(ref["rows"][-1]))
+        assert rel.max() < 1e-10, rel.max()
+    else:
+        M64 = orc.adj_rows(X.astype(np.float64), w.astype(np.float64), 0, N)
+        truth = orc.integrate(orc.Adj(0, M=M64), omega.astype(np.float64), c1, theta0.astype(np.float64), 0.0,
+                              0.01, t1, 1e-12, t_eval=te.astype(np.float64))["rows"][-1]
+        scale = np.maximum(1.0, np.abs(truth))
+        e_gpu = np.max(np.abs(g["rows"][-1].astype(np.float64) - truth) / scale)
+        e_orc = np.max(np.abs(ref["rows"][-1].astype(np.float64) - truth) / scale)
+        assert e_gpu <= 2 * e_orc + 1e-5, (e_gpu, e_orc)
+
+
+def test_sym_n50000_fp64_matches_dense_and_invariants(km):
+    cfg = PLAIN["fp64-n50000"]
+    N, K, c1 = cfg["N"], cfg["K"], cfg["c1"]
+    th, om = km.draws(km.F64, N, cfg["seed"], normal=cfg["normal"], init=cfg["init"])
+    X = km.factor(N, K, cfg["seed"], gaussian=True)
+    w = km.factor_weights(km.F64, N, K, gaussian=True)
+    print("case sym N=50000 fp64: RHS vs dense, invariants, 10 attempts", flush=True)
+    with km.Integrator(N, km.F64, atol=cfg["atol"], c1=c1) as it:
+        it.set_omega(om)
+        it.set_factor(X, w)
+        it.build_adjacency("native")
+        f_dense, cp_dense = it.rhs(th, couplings=True)
+        it.build_adjacency("sym")
+        f, cp = it.rhs(th, couplings=True)
+        # row scale sum_j |M_ij| <= sum_m |w_m| |X_im| sum_j |X_jm|
+        rs = np.abs(X) @ (np.abs(w) * np.abs(X).sum(axis=0))
+        assert np.all(np.abs(f - f_dense) <= 1e-12 * (np.abs(om) + c1 * rs))
+        assert np.all(np.abs(cp - cp_dense) <= 1e-12 * np.repeat(rs, 2))
+        assert abs(f.sum() - om.sum()) <= 1e-13 * (np.abs(f).sum() + 1.0)
+        it.set_state(th, 0.0, 0.01)
+        it.begin(np.arange(1, 11, dtype=np.float64), 10.0, keep_rows=False, ring_rows=64)
+        st = it.advance(10)
+        y, t, h = it.get_state()
+    assert st.error == 0 and st.accepted >= 5
+    drift = y.sum() - th.sum() - t * om.sum()
+    assert abs(drift) <= 1e-12 * (np.abs(y).sum() + 1.0), drift
