@@ -73,23 +73,64 @@ template <class R> __device__ __forceinline__ R wave_reduce16(R (&v)[16], int la
   return s;
 }
 
-// Wave w of the workgroup owns rows [64 w, 64 w + 64) of the block, eight
-// at a time; lane l owns the columns q 64 VEC + l VEC + e (q < NLD, e <
-// VEC): four per lane, loaded 16 bytes at a time, coalesced per row.
+// The same for 8 values: lane l returns the sum of slot 4 b5 + 2 b4 + b3.
+template <class R> __device__ __forceinline__ R wave_reduce8(R (&v)[8], int lane) {
+  const bool b5 = lane & 32, b4 = lane & 16, b3 = lane & 8;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const R keep = b5 ? v[k + 4] : v[k], send = b5 ? v[k] : v[k + 4];
+    v[k] = keep + __shfl_xor(send, 32);
+  }
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const R keep = b4 ? v[k + 2] : v[k], send = b4 ? v[k] : v[k + 2];
+    v[k] = keep + __shfl_xor(send, 16);
+  }
+  R s;
+  {
+    const R keep = b3 ? v[1] : v[0], send = b3 ? v[0] : v[1];
+    s = keep + __shfl_xor(send, 8);
+  }
+  s += __shfl_xor(s, 4);
+  s += __shfl_xor(s, 2);
+  s += __shfl_xor(s, 1);
+  return s;
+}
+
+// Wave w of the workgroup owns rows w, w + 4, w + 8, ... of the block, four
+// at a time with the next four in flight, so that the workgroup's loads at
+// any moment cover one contiguous stretch of the block (6% faster than four
+// separate quarter-blocks per workgroup); lane l owns the columns
+// q 64 VEC + l VEC + e (q < NLD, e < VEC): four per lane, loaded 16 bytes
+// at a time, coalesced per row.
 template <class R>
 __global__ __launch_bounds__(kBlock) void sym_tile_kernel(const SymArgs<R> a) {
   if (a.v.check_state && (!a.v.st->active || a.v.st->commit_pending)) return;
   constexpr int VEC = 16 / (int)sizeof(R);
   constexpr int NLD = kSymT / (kWave * VEC);
   constexpr int NC = NLD * VEC;
-  constexpr int RB = 8;
+  constexpr int RB = 4;
   constexpr int NW = kBlock / kWave;
   constexpr int RW = kSymT / NW;
-  static_assert(NC == 4 && 2 * RB == 16 && RW % RB == 0, "block shape");
+  static_assert(NC == 4 && RW % (2 * RB) == 0, "block shape");
   __shared__ R vi[2][kSymT];
   __shared__ R colbuf[NW][2][kSymT];
   const int tid = threadIdx.x, lane = tid % kWave, wave = tid / kWave;
   const long blk = blockIdx.x;
+  // the block's rows depend on the block index only: the first batch is
+  // issued before the block's coordinates and vectors are known
+  const R *base = a.A + (size_t)blk * kSymT * kSymT + lane * VEC;
+  auto row_of = [&](int k) { return wave + NW * k; };  // the wave's k-th row
+  auto load = [&](u32x4 (&raw)[RB][NLD], int kb) {
+#pragma unroll
+    for (int r = 0; r < RB; ++r)
+#pragma unroll
+      for (int q = 0; q < NLD; ++q)
+        raw[r][q] = __builtin_nontemporal_load(
+            reinterpret_cast<const u32x4 *>(base + (size_t)row_of(kb + r) * kSymT + q * kWave * VEC));
+  };
+  u32x4 rawA[RB][NLD], rawB[RB][NLD];
+  load(rawA, 0);
   const int2 ij = a.ij[blk];
   const long I = ij.x, J = ij.y, N = a.v.N;
   const R *cosv = a.v.cs_in, *sinv = a.v.cs_in + a.v.ldv;
@@ -111,23 +152,17 @@ __global__ __launch_bounds__(kBlock) void sym_tile_kernel(const SymArgs<R> a) {
   R colC[NC], colS[NC];
 #pragma unroll
   for (int k = 0; k < NC; ++k) { colC[k] = 0; colS[k] = 0; }
-  const R *base = a.A + (size_t)blk * kSymT * kSymT + lane * VEC;
-  const int r0 = wave * RW;
-  for (int rb = 0; rb < RW; rb += RB) {
-    u32x4 raw[RB][NLD];
-#pragma unroll
-    for (int r = 0; r < RB; ++r)
-#pragma unroll
-      for (int q = 0; q < NLD; ++q)
-        raw[r][q] = __builtin_nontemporal_load(
-            reinterpret_cast<const u32x4 *>(base + (size_t)(r0 + rb + r) * kSymT + q * kWave * VEC));
+  // the wave's rows kb .. kb + RB - 1: row sums reduced over the wave,
+  // column sums kept per lane
+  auto process = [&](const u32x4 (&raw)[RB][NLD], int kb) {
     R part[2 * RB];
 #pragma unroll
     for (int r = 0; r < RB; ++r) {
       R m[NC];
 #pragma unroll
       for (int q = 0; q < NLD; ++q) unpack<R>(raw[r][q], m + q * VEC);
-      const R ci = vi[0][r0 + rb + r], si = vi[1][r0 + rb + r];
+      const int row = row_of(kb + r);
+      const R ci = vi[0][row], si = vi[1][row];
       R pc = m[0] * cj[0], ps = m[0] * sj[0];
 #pragma unroll
       for (int k = 1; k < NC; ++k) {
@@ -142,12 +177,20 @@ __global__ __launch_bounds__(kBlock) void sym_tile_kernel(const SymArgs<R> a) {
       part[r] = pc;
       part[RB + r] = ps;
     }
-    const R tot = wave_reduce16<R>(part, lane);
-    if ((lane & 3) == 0) {
-      const int slot = ((lane >> 5) & 1) * 8 + ((lane >> 4) & 1) * 4 + ((lane >> 3) & 1) * 2 + ((lane >> 2) & 1);
-      R *dst = reinterpret_cast<R *>(a.prow + (size_t)blk * kSymT + r0 + rb + (slot % RB));
+    const R tot = wave_reduce8<R>(part, lane);
+    if ((lane & 7) == 0) {
+      const int slot = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
+      R *dst = reinterpret_cast<R *>(a.prow + (size_t)blk * kSymT + row_of(kb + (slot % RB)));
       dst[slot / RB] = tot;
     }
+  };
+  // two row batches in flight: the next one loads while this one is reduced
+#pragma unroll 1
+  for (int kb = 0; kb < RW; kb += 2 * RB) {
+    load(rawB, kb + RB);
+    process(rawA, kb);
+    if (kb + 2 * RB < RW) load(rawA, kb + 2 * RB);
+    process(rawB, kb + RB);
   }
   if (I == J) return;  // the diagonal block is stored whole: its row sums are complete
 #pragma unroll
