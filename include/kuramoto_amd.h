@@ -192,15 +192,16 @@ int kmt_bandwidth_probe(int device, size_t bytes, int reps, double *copy_gbps, d
  * values, may be NULL) receives the average ms of each variant, index
  * 4*r + 2*nt + balanced with r = 0, 1, 2 for 4, 8, 16 rows.  kmt_begin runs
  * it once by itself for N >= 4096.  The variants differ in speed only: every
- * row is summed in the same order.  No reference counterpart (cuBLAS
- * chooses its own gemv kernel). */
+ * row is summed in the same order.  KMT_ADJ_SYM has one form and is not
+ * tuned.  No reference counterpart (cuBLAS chooses its own gemv kernel). */
 int kmt_autotune(kmt_ctx *ctx, int reps, double *variant_ms);
 /* Pin the dense stage-kernel variant instead of the autotuner's choice:
  * rows per group 4, 8 or 16; variant_flags as kmt_layout reports them. */
 int kmt_select_variant(kmt_ctx *ctx, int rows_per_block, int variant_flags);
 
 /* Sizes the library chose (for roofline bookkeeping): padded row stride of
- * the adjacency in elements, bytes per stored element, rows per group of the
+ * the adjacency in elements (KMT_ADJ_SYM: the block edge, 256), bytes per
+ * stored element, rows per group of the
  * stage kernel, and variant_flags (bit 0: non-temporal adjacency loads, bit
  * 1: balanced grid).  Any pointer may be NULL. */
 int kmt_layout(kmt_ctx *ctx, long *lda, int *elem_bytes, int *rows_per_block, int *variant_flags);
