@@ -16,9 +16,10 @@
 //                      (column-sum partials of blocks (L, K), L < K, then
 //                      row-sum partials of blocks (K, L), L >= K), then the
 //                      stage epilogue of the dense path (row_epilogue).
-// The partials cost 4 kSymT values per block (3% of the block's bytes at
-// kSymT = 256).  Sums run in a fixed order: results are deterministic, and
-// differ from the dense path only by the order of the additions.
+// The partials cost 4 kSymT values per block (1.6% of the block's bytes at
+// kSymT = 256), written once and read once.  Sums run in a fixed order:
+// results are deterministic, and differ from the dense path only by the
+// order of the additions.
 #pragma once
 #include "kmt_kernels.hpp"
 
