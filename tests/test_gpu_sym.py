@@ -159,3 +159,37 @@ def test_sym_n50000_fp64_matches_dense_and_invariants(km):
     assert st.error == 0 and st.accepted >= 5
     drift = y.sum() - th.sum() - t * om.sum()
     assert abs(drift) <= 1e-12 * (np.abs(y).sum() + 1.0), drift
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_sym_zero_rank_and_resume(km, prec):
+    """K = 0: the stored triangle is zero, so f = omega exactly (closed
+    form).  K = 5: 9 attempts in one context equal 4 attempts, a new context
+    started from kmt_get_state, and 5 more, bit for bit (the restart path of
+    kuramoto.cu:471-494 with symmetric storage)."""
+    p = km.F32 if prec == "f32" else km.F64
+    N = 700
+    dtype, theta0, omega, X, w = _factor_case(km, p, N, 13)
+    print(f"case sym K=0 and resume {prec}", flush=True)
+    with km.Integrator(N, p, c1=3.0) as it:
+        it.set_omega(omega)
+        it.set_factor(np.zeros((N, 0), dtype), np.zeros(0, dtype))
+        it.build_adjacency("sym")
+        assert np.array_equal(it.rhs(theta0), omega)
+    te = np.arange(1, 11, dtype=np.float64)
+
+    def run(y0, t0, h0, attempts):
+        with km.Integrator(N, p, atol=1e-6 if prec == "f32" else 1e-9, c1=3.0) as it:
+            it.set_omega(omega)
+            it.set_factor(X, w)
+            it.build_adjacency("sym")
+            it.set_state(y0, t0, h0)
+            it.begin(te[te > t0], 10.0, keep_rows=False, ring_rows=64)
+            st = it.advance(attempts)
+            assert st.error == 0 and st.active == 1
+            return it.get_state()
+
+    y_full, t_full, h_full = run(theta0, 0.0, 0.01, 9)
+    y1, t1, h1 = run(theta0, 0.0, 0.01, 4)
+    y2, t2, h2 = run(y1, t1, h1, 5)
+    assert t2 == t_full and h2 == h_full and np.array_equal(y2, y_full)
