@@ -314,9 +314,9 @@ def main():
     if cfg["storage"] == "factored":
         # per RHS: X read twice (N K s_y each) + theta-derived vectors in, k out, omega
         alg_bytes = 2 * N * cfg["K"] * s_y + N * (2 * s_y + s_y)
-    elif cfg["storage"] == "sym":
+    elif cfg["storage"].startswith("sym"):
         # a symmetric matrix carries N (N + 1) / 2 distinct entries
-        alg_bytes = s_y * N * (N + 1) // 2 + N * (2 * s_y + s_y)
+        alg_bytes = r["elem_bytes"] * N * (N + 1) // 2 + N * (2 * s_y + s_y)
     else:
         alg_bytes = r["elem_bytes"] * N * N + N * (2 * s_y + s_y)
     achieved = alg_bytes / (r["stage_ms"] / 1e3)
@@ -348,7 +348,7 @@ def main():
                                + ("low-rank factor only (-A)" if cfg["storage"] == "factored" else
                                   "-A, tiles rebuilt in registers" if cfg["storage"] == "rebuild" else
                                   f"symmetric adjacency, upper block triangle ({r['elem_bytes']} B/elem)"
-                                  if cfg["storage"] == "sym" else
+                                  if cfg["storage"].startswith("sym") else
                                   f"dense adjacency {cfg['storage']} ({r['elem_bytes']} B/elem)")
                                + f", atol={cfg['atol']}, c={cfg['c1']}",
                    "N": N, "accepted_in_region": accepted,
@@ -357,15 +357,15 @@ def main():
                    "t_after": r["st"].t, "adjacency_build_s": round(r["gen_s"], 3)},
         "roofline": {**roof,
                      "traffic": profiled_traffic(args.config, "lowrank_stage_kernel" if cfg["storage"] == "factored"
-                                                 else "sym_tile_kernel" if cfg["storage"] == "sym"
+                                                 else "sym_tile_kernel" if cfg["storage"].startswith("sym")
                                                  else "stage_kernel"),
                      "kernel": ("lowrank_stage_kernel (+ gram at stage 2; one RHS)" if cfg["storage"] == "factored"
                                 else "rebuild_stage_kernel (tiles rebuilt in registers)" if cfg["storage"] == "rebuild"
                                 else "sym_tile_kernel + sym_reduce_kernel (one RHS from the upper block triangle)"
-                                if cfg["storage"] == "sym"
+                                if cfg["storage"].startswith("sym")
                                 else "stage_kernel (fused gemv-pair + DP45 stage epilogue)"),
                      "bytes_per_launch": alg_bytes, "avg_launch_ms": r["stage_ms"],
-                     "variant": ({"block_edge": r["lda"]} if cfg["storage"] == "sym" else
+                     "variant": ({"block_edge": r["lda"]} if cfg["storage"].startswith("sym") else
                                  {"rows_per_group": r["rows"], "nontemporal_loads": r["nt"],
                                   "balanced_grid": r["balanced"]}),
                      "measured_read_stream_gbps": r["read_gbps"], "measured_copy_gbps": r["copy_gbps"],

@@ -43,8 +43,10 @@ extern "C" {
 #define KMT_ADJ_NATIVE 0 /* same type as the working precision */
 #define KMT_ADJ_BF16 2   /* bfloat16 with FP32 accumulation (KMT_F32 only) */
 #define KMT_ADJ_F16 3    /* IEEE half with FP32 accumulation (KMT_F32 only) */
-#define KMT_ADJ_SYM 4    /* working precision, upper block triangle of a symmetric
-                            matrix (half the bytes per RHS); no reference
+#define KMT_ADJ_SYM 4    /* flag: only the upper block triangle of a symmetric
+                            matrix (half the bytes per RHS), in the working
+                            precision (KMT_ADJ_SYM) or in 16 bits
+                            (KMT_ADJ_SYM | KMT_ADJ_BF16 / _F16); no reference
                             counterpart: its gemvs read the full matrix */
 
 typedef struct kmt_ctx kmt_ctx;

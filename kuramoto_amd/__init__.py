@@ -22,7 +22,8 @@ HEADER = os.path.join(os.path.dirname(_HERE), "include", "kuramoto_amd.h")
 
 F32, F64 = 0, 1
 ADJ_NATIVE, ADJ_BF16, ADJ_F16, ADJ_SYM = 0, 2, 3, 4
-STORAGE = {"native": ADJ_NATIVE, "bf16": ADJ_BF16, "f16": ADJ_F16, "sym": ADJ_SYM}
+STORAGE = {"native": ADJ_NATIVE, "bf16": ADJ_BF16, "f16": ADJ_F16, "sym": ADJ_SYM,
+           "sym-bf16": ADJ_SYM | ADJ_BF16, "sym-f16": ADJ_SYM | ADJ_F16}
 
 ERRORS = {-1: "KMT_ERR_ARG", -2: "KMT_ERR_HIP", -3: "KMT_ERR_STATE", -4: "KMT_ERR_NUMERIC", -5: "KMT_ERR_MEMORY"}
 

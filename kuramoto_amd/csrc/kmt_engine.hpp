@@ -179,8 +179,8 @@ template <class R> struct Engine : EngineBase {
   }
 
   int layout(long *plda, int *eb, int *rows, int *flags) override {
-    if (plda) *plda = storage == KMT_ADJ_SYM ? kSymT : lda;
-    if (eb) *eb = storage == KMT_ADJ_BF16 || storage == KMT_ADJ_F16 ? 2 : (int)sizeof(R);
+    if (plda) *plda = sym() ? kSymT : lda;
+    if (eb) *eb = storage < 0 ? (int)sizeof(R) : (int)elem_bytes(storage);
     if (rows) *rows = tune_rows;
     if (flags) *flags = (tune_nt ? 1 : 0) | (tune_bal ? 2 : 0);
     return KMT_OK;
@@ -249,7 +249,12 @@ template <class R> struct Engine : EngineBase {
   }
 
   // ---------------- adjacency ----------------
-  size_t elem_bytes(int stg) const { return (stg == KMT_ADJ_BF16 || stg == KMT_ADJ_F16) ? 2 : sizeof(R); }
+  size_t elem_bytes(int stg) const {
+    const int el = stg & ~KMT_ADJ_SYM;
+    return (el == KMT_ADJ_BF16 || el == KMT_ADJ_F16) ? 2 : sizeof(R);
+  }
+  static bool is_sym(int stg) { return stg >= 0 && (stg & KMT_ADJ_SYM); }
+  bool sym() const { return is_sym(storage); }
 
   // block layout of KMT_ADJ_SYM: the (I, J) table and the partial buffers
   int ensure_sym() {
@@ -261,6 +266,7 @@ template <class R> struct Engine : EngineBase {
       for (long J = I; J < sym_nb; ++J) ij[(size_t)sym_index(I, J, sym_nb)] = make_int2((int)I, (int)J);
     HIP_TRY(hipMalloc((void **)&sym_ij, sizeof(int2) * ij.size()));
     HIP_TRY(hipMemcpyAsync(sym_ij, ij.data(), sizeof(int2) * ij.size(), hipMemcpyHostToDevice, stream));
+    // sized for the wider of the two accumulation types (R, float)
     HIP_TRY(hipMalloc((void **)&sym_prow, sizeof(SymPair<R>) * (size_t)sym_blocks * kSymT));
     HIP_TRY(hipMalloc((void **)&sym_pcol, sizeof(SymPair<R>) * (size_t)sym_blocks * kSymT));
     HIP_TRY(hipMemsetAsync(sym_prow, 0, sizeof(SymPair<R>) * (size_t)sym_blocks * kSymT, stream));
@@ -270,11 +276,12 @@ template <class R> struct Engine : EngineBase {
   }
 
   int ensure_A(int stg) {
-    if (stg != KMT_ADJ_NATIVE && stg != KMT_ADJ_BF16 && stg != KMT_ADJ_F16 && stg != KMT_ADJ_SYM)
+    const int el = stg & ~KMT_ADJ_SYM;
+    if (stg < 0 || (el != KMT_ADJ_NATIVE && el != KMT_ADJ_BF16 && el != KMT_ADJ_F16))
       return fail(KMT_ERR_ARG, "unknown adjacency storage %d", stg);
-    if (stg == KMT_ADJ_SYM) KMT_TRY(ensure_sym());
-    const size_t bytes = stg == KMT_ADJ_SYM ? (size_t)sym_blocks * kSymT * kSymT * sizeof(R)
-                                            : (size_t)rows_alloc * (size_t)lda * elem_bytes(stg);
+    if (is_sym(stg)) KMT_TRY(ensure_sym());
+    const size_t bytes = is_sym(stg) ? (size_t)sym_blocks * kSymT * kSymT * elem_bytes(stg)
+                                     : (size_t)rows_alloc * (size_t)lda * elem_bytes(stg);
     if (A && A_bytes == bytes && storage == stg) return KMT_OK;
     drop_graphs();
     if (A) {
@@ -329,16 +336,23 @@ template <class R> struct Engine : EngineBase {
     if (src_prec != (sizeof(R) == 4 ? KMT_F32 : KMT_F64))
       return fail(KMT_ERR_ARG, "adjacency source precision must match the context precision");
     const R *M = (const R *)Mv;
-    if (stg == KMT_ADJ_SYM) {
+    if (is_sym(stg)) {
       for (long i = 0; i < N; ++i)
         for (long j = i + 1; j < N; ++j)
           if (!(M[(size_t)i * ld + j] == M[(size_t)j * ld + i]) &&
               !(std::isnan((double)M[(size_t)i * ld + j]) && std::isnan((double)M[(size_t)j * ld + i])))
             return fail(KMT_ERR_ARG, "KMT_ADJ_SYM needs a symmetric matrix: M[%ld][%ld] != M[%ld][%ld]", i, j, j, i);
       KMT_TRY(ensure_A(stg));
-      mscale_ = 1.0;
       factored = 0;
-      return upload_sym(M, ld);
+      mscale_ = 1.0;
+      if ((stg & ~KMT_ADJ_SYM) == KMT_ADJ_NATIVE) return upload_sym<R>(M, ld, (R)1);
+      double mx = 0;
+      for (long i = 0; i < N; ++i)
+        for (long j = 0; j < N; ++j) mx = std::max(mx, std::fabs((double)M[(size_t)i * ld + j]));
+      mscale_ = pow2_at_least(mx);
+      const R inv = (R)(1.0 / mscale_);
+      if ((stg & ~KMT_ADJ_SYM) == KMT_ADJ_BF16) return upload_sym<bf16_t>(M, ld, inv);
+      return upload_sym<f16_t>(M, ld, inv);
     }
     KMT_TRY(ensure_A(stg));
     if (stg == KMT_ADJ_NATIVE) {
@@ -358,10 +372,13 @@ template <class R> struct Engine : EngineBase {
     return upload_rows<f16_t>(M, ld, inv);
   }
 
-  // pack the upper block triangle of a host matrix, a few blocks per copy
-  int upload_sym(const R *M, long ld) {
+  // pack the upper block triangle of a host matrix, a few blocks per copy,
+  // through a device staging buffer converted into the stored type
+  template <class S> int upload_sym(const R *M, long ld, R inv_scale) {
     const long per = 32;
     std::vector<R> buf((size_t)per * kSymT * kSymT);
+    R *stage = nullptr;
+    HIP_TRY(hipMalloc((void **)&stage, sizeof(R) * buf.size()));
     for (long b0 = 0; b0 < sym_blocks; b0 += per) {
       const long nbk = std::min(per, sym_blocks - b0);
       std::fill(buf.begin(), buf.end(), (R)0);
@@ -375,10 +392,21 @@ template <class R> struct Engine : EngineBase {
             for (long c = 0; c < kSymT && J * kSymT + c < N; ++c)
               dst[(size_t)r * kSymT + c] = M[(size_t)(I * kSymT + r) * ld + J * kSymT + c];
         }
-      HIP_TRY(hipMemcpyAsync((R *)A + (size_t)b0 * kSymT * kSymT, buf.data(), sizeof(R) * (size_t)nbk * kSymT * kSymT,
-                             hipMemcpyHostToDevice, stream));
-      HIP_TRY(hipStreamSynchronize(stream));  // buf is refilled next
+      hipError_t e = hipMemcpyAsync(stage, buf.data(), sizeof(R) * (size_t)nbk * kSymT * kSymT, hipMemcpyHostToDevice,
+                                    stream);
+      if (e == hipSuccess) {
+        dim3 grid(1, (unsigned)(nbk * kSymT));
+        convert_rows_kernel<R, S><<<grid, kBlock, 0, stream>>>(stage, kSymT, kSymT, nbk * kSymT,
+                                                               (S *)A + (size_t)b0 * kSymT * kSymT, kSymT, inv_scale);
+        e = hipGetLastError();
+      }
+      if (e == hipSuccess) e = hipStreamSynchronize(stream);  // buf is refilled next
+      if (e != hipSuccess) {
+        (void)hipFree(stage);
+        return fail(KMT_ERR_HIP, "adjacency upload failed: %s", hipGetErrorString(e));
+      }
     }
+    HIP_TRY(hipFree(stage));
     return KMT_OK;
   }
 
@@ -436,7 +464,7 @@ template <class R> struct Engine : EngineBase {
     }
     if (stg == KMT_ADJ_SYM) {
       mscale_ = 1.0;
-      gen_adj_sym_kernel<R><<<(unsigned)sym_blocks, kBlock, 0, stream>>>(X, w, K, N, sym_ij, (R *)A);
+      gen_adj_sym_kernel<R, R><<<(unsigned)sym_blocks, kBlock, 0, stream>>>(X, w, K, N, sym_ij, (R *)A, (R)1);
       HIP_TRY(hipGetLastError());
       HIP_TRY(hipStreamSynchronize(stream));
       return KMT_OK;
@@ -450,6 +478,15 @@ template <class R> struct Engine : EngineBase {
     }
     mscale_ = pow2_at_least(mx);
     const R inv = (R)(1.0 / mscale_);
+    if (is_sym(stg)) {
+      if ((stg & ~KMT_ADJ_SYM) == KMT_ADJ_BF16)
+        gen_adj_sym_kernel<R, bf16_t><<<(unsigned)sym_blocks, kBlock, 0, stream>>>(X, w, K, N, sym_ij, (bf16_t *)A, inv);
+      else
+        gen_adj_sym_kernel<R, f16_t><<<(unsigned)sym_blocks, kBlock, 0, stream>>>(X, w, K, N, sym_ij, (f16_t *)A, inv);
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipStreamSynchronize(stream));
+      return KMT_OK;
+    }
     if (stg == KMT_ADJ_BF16) return gen_rows<bf16_t>(inv);
     return gen_rows<f16_t>(inv);
   }
@@ -484,7 +521,11 @@ template <class R> struct Engine : EngineBase {
       dim3 grid((unsigned)std::min<long>((N + kBlock - 1) / kBlock, 64), (unsigned)nr);
       hipError_t e = hipSuccess;
       if (storage == KMT_ADJ_SYM)
-        read_sym_rows_kernel<R><<<grid, kBlock, 0, stream>>>((const R *)A, sym_nb, N, r0, nr, stage);
+        read_sym_rows_kernel<R, R><<<grid, kBlock, 0, stream>>>((const R *)A, sym_nb, N, r0, nr, (R)1, stage);
+      else if (storage == (KMT_ADJ_SYM | KMT_ADJ_BF16))
+        read_sym_rows_kernel<R, bf16_t><<<grid, kBlock, 0, stream>>>((const bf16_t *)A, sym_nb, N, r0, nr, (R)mscale_, stage);
+      else if (storage == (KMT_ADJ_SYM | KMT_ADJ_F16))
+        read_sym_rows_kernel<R, f16_t><<<grid, kBlock, 0, stream>>>((const f16_t *)A, sym_nb, N, r0, nr, (R)mscale_, stage);
       else if (storage == KMT_ADJ_NATIVE)
         read_rows_kernel<R, R><<<grid, kBlock, 0, stream>>>((const R *)A + (size_t)r0 * lda, lda, N, nr, (R)1, stage);
       else if (storage == KMT_ADJ_BF16)
@@ -562,20 +603,27 @@ template <class R> struct Engine : EngineBase {
   }
 
   // KMT_ADJ_SYM: block kernel over the stored triangle, then the row reduction
-  template <int KIND> void launch_sym(const Vecs<R> &v) {
-    SymArgs<R> a;
-    a.A = (const R *)A;
+  template <class S, int KIND> void launch_sym_as(const Vecs<R> &v) {
+    using Acc = typename Store<S>::Acc;
+    SymArgs<R, S> a;
+    a.A = (const S *)A;
     a.ij = sym_ij;
     a.nb = sym_nb;
-    a.prow = sym_prow;
-    a.pcol = sym_pcol;
+    a.prow = reinterpret_cast<SymPair<Acc> *>(sym_prow);
+    a.pcol = reinterpret_cast<SymPair<Acc> *>(sym_pcol);
+    a.mscale = (R)mscale_;
     a.v = v;
-    sym_tile_kernel<R><<<(unsigned)sym_blocks, kBlock, 0, stream>>>(a);
-    sym_reduce_kernel<R, KIND><<<(unsigned)sym_nb, kBlock, 0, stream>>>(a);
+    sym_tile_kernel<R, S><<<(unsigned)sym_blocks, kBlock, 0, stream>>>(a);
+    sym_reduce_kernel<R, S, KIND><<<(unsigned)sym_nb, kBlock, 0, stream>>>(a);
+  }
+  template <int KIND> void launch_sym(const Vecs<R> &v) {
+    if (storage == KMT_ADJ_SYM) launch_sym_as<R, KIND>(v);
+    else if (storage == (KMT_ADJ_SYM | KMT_ADJ_BF16)) launch_sym_as<bf16_t, KIND>(v);
+    else launch_sym_as<f16_t, KIND>(v);
   }
 
   template <int KIND> void launch_dense(const Vecs<R> &v) {
-    if (storage == KMT_ADJ_SYM) launch_sym<KIND>(v);
+    if (sym()) launch_sym<KIND>(v);
     else if (storage == KMT_ADJ_NATIVE) launch_dense_kind<R, KIND>(v);
     else if (storage == KMT_ADJ_BF16) launch_dense_kind<bf16_t, KIND>(v);
     else launch_dense_kind<f16_t, KIND>(v);
@@ -679,7 +727,7 @@ template <class R> struct Engine : EngineBase {
   int n_parts_stage7() const {
     if (factored == KMT_LOWRANK_REBUILD) return (int)((N + rebuild_rows() - 1) / rebuild_rows());
     if (factored) return lr_fused() ? lr_grid : (int)((N + kBlock - 1) / kBlock);
-    if (storage == KMT_ADJ_SYM) return (int)sym_nb;
+    if (sym()) return (int)sym_nb;
     if (tune_bal) return bal_grid;
     return (int)((N + tune_rows - 1) / tune_rows);
   }
@@ -691,7 +739,7 @@ template <class R> struct Engine : EngineBase {
   // Every variant sums each row in the same order, so the choice changes
   // speed only, never results.
   int autotune(int reps) {
-    if (factored || !A || storage == KMT_ADJ_SYM) return KMT_OK;  // one form of the symmetric kernels
+    if (factored || !A || sym()) return KMT_OK;  // one form of the symmetric kernels
     if (!k_scr) {
       HIP_TRY(hipMalloc((void **)&k_scr, sizeof(R) * 7 * ldv));
       HIP_TRY(hipMemsetAsync(k_scr, 0, sizeof(R) * 7 * ldv, stream));

@@ -1,4 +1,6 @@
-// Symmetric adjacency stored as its upper block triangle (KMT_ADJ_SYM).
+// Symmetric adjacency stored as its upper block triangle (KMT_ADJ_SYM, in
+// the working precision or, with KMT_ADJ_BF16 / KMT_ADJ_F16, in 16 bits
+// with a power-of-two scale and FP32 accumulation as in the dense path).
 //
 // Every adjacency generated from a factor is exactly symmetric (M_ij and
 // M_ji are the same products, kuramoto.cu:145-161), so half of the dense
@@ -21,6 +23,8 @@
 // results are deterministic, and differ from the dense path only by the
 // order of the additions.
 #pragma once
+#include <type_traits>
+
 #include "kmt_kernels.hpp"
 
 namespace kmt {
@@ -31,18 +35,40 @@ __host__ __device__ __forceinline__ long sym_index(long I, long J, long nb) {
   return I * nb - I * (I - 1) / 2 + (J - I);
 }
 
-template <class R> struct alignas(2 * sizeof(R)) SymPair {
-  R c, s;
+template <class T> struct alignas(2 * sizeof(T)) SymPair {
+  T c, s;
 };
 
-template <class R> struct SymArgs {
-  const R *A;       // stored blocks
-  const int2 *ij;   // (I, J) of every stored block
-  long nb;          // blocks per row / column of the padded matrix
-  SymPair<R> *prow; // [block][kSymT]: row sums (C, S) of the block
-  SymPair<R> *pcol; // [block][kSymT]: column sums of off-diagonal blocks
+// R: working precision; S: stored element; Acc: accumulation (R for native
+// storage, float for 16-bit storage).
+template <class R, class S> struct SymArgs {
+  using Acc = typename Store<S>::Acc;
+  const S *A;          // stored blocks
+  const int2 *ij;      // (I, J) of every stored block
+  long nb;             // blocks per row / column of the padded matrix
+  SymPair<Acc> *prow;  // [block][kSymT]: row sums (C, S) of the block
+  SymPair<Acc> *pcol;  // [block][kSymT]: column sums of off-diagonal blocks
+  R mscale;            // M = mscale * stored
   Vecs<R> v;
 };
+
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+// One load of a lane: 16 bytes of 4- and 8-byte elements, 8 bytes (four
+// values) of 16-bit ones.
+template <class S> struct SymLoad {
+  static constexpr int BYTES = sizeof(S) >= 4 ? 16 : 8;
+  static constexpr int VEC = BYTES / (int)sizeof(S);
+  using Raw = typename std::conditional<BYTES == 16, u32x4, u32x2>::type;
+};
+template <class S> __device__ __forceinline__ void sym_unpack(const u32x4 v, typename Store<S>::Acc *o) { unpack<S>(v, o); }
+template <class S> __device__ __forceinline__ void sym_unpack(const u32x2 v, float *o);
+template <> __device__ __forceinline__ void sym_unpack<bf16_t>(const u32x2 v, float *o) {
+  o[0] = bf16_lo(v.x); o[1] = bf16_hi(v.x); o[2] = bf16_lo(v.y); o[3] = bf16_hi(v.y);
+}
+template <> __device__ __forceinline__ void sym_unpack<f16_t>(const u32x2 v, float *o) {
+  o[0] = f16_lo(v.x); o[1] = f16_hi(v.x); o[2] = f16_lo(v.y); o[3] = f16_hi(v.y);
+}
 
 // The 16 values v[0..15] of every lane, each summed over the 64 lanes of
 // the wave; lane l returns the sum of slot 8 b5 + 4 b4 + 2 b3 + b2 (b = bits
@@ -104,84 +130,86 @@ template <class R> __device__ __forceinline__ R wave_reduce8(R (&v)[8], int lane
 // separate quarter-blocks per workgroup); lane l owns the columns
 // q 64 VEC + l VEC + e (q < NLD, e < VEC): four per lane, loaded 16 bytes
 // at a time, coalesced per row.
-template <class R>
-__global__ __launch_bounds__(kBlock) void sym_tile_kernel(const SymArgs<R> a) {
+template <class R, class S>
+__global__ __launch_bounds__(kBlock) void sym_tile_kernel(const SymArgs<R, S> a) {
   if (a.v.check_state && (!a.v.st->active || a.v.st->commit_pending)) return;
-  constexpr int VEC = 16 / (int)sizeof(R);
+  using Acc = typename Store<S>::Acc;
+  using Raw = typename SymLoad<S>::Raw;
+  constexpr int VEC = SymLoad<S>::VEC;
   constexpr int NLD = kSymT / (kWave * VEC);
   constexpr int NC = NLD * VEC;
   constexpr int RB = 4;
   constexpr int NW = kBlock / kWave;
   constexpr int RW = kSymT / NW;
   static_assert(NC == 4 && RW % (2 * RB) == 0, "block shape");
-  __shared__ R vi[2][kSymT];
-  __shared__ R colbuf[NW][2][kSymT];
+  __shared__ Acc vi[2][kSymT];
+  __shared__ Acc colbuf[NW][2][kSymT];
   const int tid = threadIdx.x, lane = tid % kWave, wave = tid / kWave;
   const long blk = blockIdx.x;
   // the block's rows depend on the block index only: the first batch is
   // issued before the block's coordinates and vectors are known
-  const R *base = a.A + (size_t)blk * kSymT * kSymT + lane * VEC;
+  const S *base = a.A + (size_t)blk * kSymT * kSymT + lane * VEC;
   auto row_of = [&](int k) { return wave + NW * k; };  // the wave's k-th row
-  auto load = [&](u32x4 (&raw)[RB][NLD], int kb) {
+  auto load = [&](Raw (&raw)[RB][NLD], int kb) {
 #pragma unroll
     for (int r = 0; r < RB; ++r)
 #pragma unroll
       for (int q = 0; q < NLD; ++q)
         raw[r][q] = __builtin_nontemporal_load(
-            reinterpret_cast<const u32x4 *>(base + (size_t)row_of(kb + r) * kSymT + q * kWave * VEC));
+            reinterpret_cast<const Raw *>(base + (size_t)row_of(kb + r) * kSymT + q * kWave * VEC));
   };
-  u32x4 rawA[RB][NLD], rawB[RB][NLD];
+  Raw rawA[RB][NLD], rawB[RB][NLD];
   load(rawA, 0);
   const int2 ij = a.ij[blk];
   const long I = ij.x, J = ij.y, N = a.v.N;
   const R *cosv = a.v.cs_in, *sinv = a.v.cs_in + a.v.ldv;
   {
     const long i = I * kSymT + tid;
-    vi[0][tid] = i < N ? cosv[i] : (R)0;
-    vi[1][tid] = i < N ? sinv[i] : (R)0;
+    vi[0][tid] = i < N ? (Acc)cosv[i] : (Acc)0;
+    vi[1][tid] = i < N ? (Acc)sinv[i] : (Acc)0;
   }
-  R cj[NC], sj[NC];
+  Acc cj[NC], sj[NC];
 #pragma unroll
   for (int q = 0; q < NLD; ++q)
 #pragma unroll
     for (int e = 0; e < VEC; ++e) {
       const long j = J * kSymT + q * kWave * VEC + lane * VEC + e;
-      cj[q * VEC + e] = j < N ? cosv[j] : (R)0;
-      sj[q * VEC + e] = j < N ? sinv[j] : (R)0;
+      cj[q * VEC + e] = j < N ? (Acc)cosv[j] : (Acc)0;
+      sj[q * VEC + e] = j < N ? (Acc)sinv[j] : (Acc)0;
     }
   __syncthreads();
-  R colC[NC], colS[NC];
+  Acc colC[NC], colS[NC];
 #pragma unroll
   for (int k = 0; k < NC; ++k) { colC[k] = 0; colS[k] = 0; }
   // the wave's rows kb .. kb + RB - 1: row sums reduced over the wave,
   // column sums kept per lane
-  auto process = [&](const u32x4 (&raw)[RB][NLD], int kb) {
-    R part[2 * RB];
+  auto process = [&](const Raw (&raw)[RB][NLD], int kb) {
+    Acc part[2 * RB];
 #pragma unroll
     for (int r = 0; r < RB; ++r) {
-      R m[NC];
+      Acc m[NC];
 #pragma unroll
-      for (int q = 0; q < NLD; ++q) unpack<R>(raw[r][q], m + q * VEC);
+      for (int q = 0; q < NLD; ++q) sym_unpack<S>(raw[r][q], m + q * VEC);
       const int row = row_of(kb + r);
-      const R ci = vi[0][row], si = vi[1][row];
-      R pc = m[0] * cj[0], ps = m[0] * sj[0];
+      const Acc ci = vi[0][row], si = vi[1][row];
+      Acc pc = m[0] * cj[0], ps = m[0] * sj[0];
 #pragma unroll
       for (int k = 1; k < NC; ++k) {
-        pc = rfma<R>(m[k], cj[k], pc);
-        ps = rfma<R>(m[k], sj[k], ps);
+        pc = rfma<Acc>(m[k], cj[k], pc);
+        ps = rfma<Acc>(m[k], sj[k], ps);
       }
 #pragma unroll
       for (int k = 0; k < NC; ++k) {
-        colC[k] = rfma<R>(m[k], ci, colC[k]);
-        colS[k] = rfma<R>(m[k], si, colS[k]);
+        colC[k] = rfma<Acc>(m[k], ci, colC[k]);
+        colS[k] = rfma<Acc>(m[k], si, colS[k]);
       }
       part[r] = pc;
       part[RB + r] = ps;
     }
-    const R tot = wave_reduce8<R>(part, lane);
+    const Acc tot = wave_reduce8<Acc>(part, lane);
     if ((lane & 7) == 0) {
       const int slot = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
-      R *dst = reinterpret_cast<R *>(a.prow + (size_t)blk * kSymT + row_of(kb + (slot % RB)));
+      Acc *dst = reinterpret_cast<Acc *>(a.prow + (size_t)blk * kSymT + row_of(kb + (slot % RB)));
       dst[slot / RB] = tot;
     }
   };
@@ -203,7 +231,7 @@ __global__ __launch_bounds__(kBlock) void sym_tile_kernel(const SymArgs<R> a) {
       colbuf[wave][1][c] = colS[q * VEC + e];
     }
   __syncthreads();
-  SymPair<R> out;
+  SymPair<Acc> out;
   out.c = colbuf[0][0][tid];
   out.s = colbuf[0][1][tid];
 #pragma unroll
@@ -216,8 +244,9 @@ __global__ __launch_bounds__(kBlock) void sym_tile_kernel(const SymArgs<R> a) {
 
 // One thread per row; a workgroup covers exactly one block row (kBlock =
 // kSymT).  Stage 7 writes one error partial per workgroup.
-template <class R, int KIND>
-__global__ __launch_bounds__(kBlock) void sym_reduce_kernel(const SymArgs<R> a) {
+template <class R, class S, int KIND>
+__global__ __launch_bounds__(kBlock) void sym_reduce_kernel(const SymArgs<R, S> a) {
+  using Acc = typename Store<S>::Acc;
   if constexpr (KIND >= 2) {
     if (a.v.check_state && (!a.v.st->active || a.v.st->commit_pending)) return;
   }
@@ -227,23 +256,23 @@ __global__ __launch_bounds__(kBlock) void sym_reduce_kernel(const SymArgs<R> a) 
   const long i = K * kSymT + r;
   double sq = 0.0;
   if (i < a.v.N) {
-    R C = 0, S = 0;
-    const SymPair<R> *pc = a.pcol + r, *pr = a.prow + r;
+    Acc C = 0, Sn = 0;
+    const SymPair<Acc> *pc = a.pcol + r, *pr = a.prow + r;
 #pragma unroll 4
     for (long L = 0; L < K; ++L) {
-      const SymPair<R> p = pc[(size_t)sym_index(L, K, a.nb) * kSymT];
+      const SymPair<Acc> p = pc[(size_t)sym_index(L, K, a.nb) * kSymT];
       C += p.c;
-      S += p.s;
+      Sn += p.s;
     }
-    const SymPair<R> *row = pr + (size_t)sym_index(K, K, a.nb) * kSymT;
+    const SymPair<Acc> *row = pr + (size_t)sym_index(K, K, a.nb) * kSymT;
 #pragma unroll 4
     for (long L = K; L < a.nb; ++L) {
-      const SymPair<R> p = row[(size_t)(L - K) * kSymT];
+      const SymPair<Acc> p = row[(size_t)(L - K) * kSymT];
       C += p.c;
-      S += p.s;
+      Sn += p.s;
     }
     const R h = (KIND >= 2) ? a.v.st->h : (R)0;
-    sq = row_epilogue<R, KIND>(a.v, i, C, S, h);
+    sq = row_epilogue<R, KIND>(a.v, i, (R)C * a.mscale, (R)Sn * a.mscale, h);
   }
   if constexpr (KIND == 7) {
     const double s = block_sum(sq, sh);
@@ -254,13 +283,13 @@ __global__ __launch_bounds__(kBlock) void sym_reduce_kernel(const SymArgs<R> a) 
 // Stored blocks from the factor with gen_adj_kernel's formula (the same
 // value bits as the dense matrix); one workgroup per block, one column per
 // thread.
-template <class R>
+template <class R, class S>
 __global__ __launch_bounds__(kBlock) void gen_adj_sym_kernel(const R *X, const R *w, int K, long N, const int2 *ij,
-                                                             R *A) {
+                                                             S *A, R inv_scale) {
   const long blk = blockIdx.x;
   const int2 b = ij[blk];
   const long j = (long)b.y * kSymT + threadIdx.x;
-  R *dst = A + (size_t)blk * kSymT * kSymT + threadIdx.x;
+  S *dst = A + (size_t)blk * kSymT * kSymT + threadIdx.x;
   const R *Xj = X + (j < N ? j : 0) * K;
   for (int r = 0; r < kSymT; ++r) {
     const long i = (long)b.x * kSymT + r;
@@ -269,22 +298,22 @@ __global__ __launch_bounds__(kBlock) void gen_adj_sym_kernel(const R *X, const R
       const R *Xi = X + i * K;
       for (int m = 0; m < K; ++m) acc = rfma<R>(w[m], Xi[m] * Xj[m], acc);
     }
-    dst[(size_t)r * kSymT] = acc;
+    dst[(size_t)r * kSymT] = store_value<R, S>(acc, inv_scale);
   }
 }
 
 // Rows [row0, row0 + nrows) of the full matrix from the stored triangle.
-template <class R>
-__global__ __launch_bounds__(kBlock) void read_sym_rows_kernel(const R *A, long nb, long N, long row0, long nrows,
-                                                               R *dst) {
+template <class R, class S>
+__global__ __launch_bounds__(kBlock) void read_sym_rows_kernel(const S *A, long nb, long N, long row0, long nrows,
+                                                               R scale, R *dst) {
   const long r = blockIdx.y;
   if (r >= nrows) return;
   const long i = row0 + r, I = i / kSymT, ri = i % kSymT;
   for (long j = (long)blockIdx.x * kBlock + threadIdx.x; j < N; j += (long)gridDim.x * kBlock) {
     const long J = j / kSymT, cj = j % kSymT;
-    const R v = I <= J ? A[((size_t)sym_index(I, J, nb) * kSymT + ri) * kSymT + cj]
+    const S v = I <= J ? A[((size_t)sym_index(I, J, nb) * kSymT + ri) * kSymT + cj]
                        : A[((size_t)sym_index(J, I, nb) * kSymT + cj) * kSymT + ri];
-    dst[(size_t)r * N + j] = v;
+    dst[(size_t)r * N + j] = load_value<R, S>(v, scale);
   }
 }
 

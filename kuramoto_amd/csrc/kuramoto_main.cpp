@@ -23,9 +23,10 @@
 //   --adj-dtype=T       HBM storage of the dense adjacency: native (default),
 //                       bf16 or f16 (FP32 accumulation of the products;
 //                       kuramoto_64 keeps the state, stage combinations and
-//                       error norm in FP64), or sym (working precision, only
-//                       the upper block triangle: half the bytes per RHS; a
-//                       reloaded adj.dat must be symmetric)
+//                       error norm in FP64), or sym, sym-bf16, sym-f16 (only
+//                       the upper block triangle, in the working precision
+//                       or in 16 bits: half the bytes per RHS; a reloaded
+//                       adj.dat must be symmetric)
 //   --rebuild           with -A: rebuild the adjacency tiles from the factor in
 //                       registers at every stage (dense semantics, O(N^2 K)
 //                       work, no N^2 storage) instead of the O(N K) factored
@@ -102,7 +103,7 @@ static void usage() {
   std::printf("atol is absolute error tolerance. Default 1e-6.\n");
   std::printf("gpu is index of the gpu. Default 0.\n");
   std::printf("filebase is base file name for output. \n");
-  std::printf("extensions: --gauss --adj-dtype=native|bf16|f16|sym --rebuild --max-steps=n --chunk=n\n");
+  std::printf("extensions: --gauss --adj-dtype=native|bf16|f16|sym|sym-bf16|sym-f16 --rebuild --max-steps=n --chunk=n\n");
 }
 
 static void append(const char *suffix, const void *data, size_t elem, size_t n) {
@@ -261,6 +262,8 @@ int main(int argc, char *argv[]) {
           else if (!std::strcmp(optarg, "bf16")) storage = KMT_ADJ_BF16;
           else if (!std::strcmp(optarg, "f16")) storage = KMT_ADJ_F16;
           else if (!std::strcmp(optarg, "sym")) storage = KMT_ADJ_SYM;
+          else if (!std::strcmp(optarg, "sym-bf16")) storage = KMT_ADJ_SYM | KMT_ADJ_BF16;
+          else if (!std::strcmp(optarg, "sym-f16")) storage = KMT_ADJ_SYM | KMT_ADJ_F16;
           else { std::fprintf(stderr, "kuramoto: unknown --adj-dtype %s\n", optarg); return 1; }
           break;
         case 1003: max_steps = atoi(optarg); break;
@@ -297,10 +300,11 @@ int main(int argc, char *argv[]) {
   size_t fr = 0, total = 0;
   CALL(kmt_mem_info(gpu, &fr, &total));
   const size_t adj_elem = (storage == KMT_ADJ_NATIVE || storage == KMT_ADJ_SYM) ? sizeof(R) : 2;
+  const bool sym = (storage & KMT_ADJ_SYM) != 0;
   // symmetric storage: 256 x 256 blocks of the upper block triangle, plus
   // two partial sums (C, S) per row of every block
   const size_t nbk = (N + 255) / 256, sym_blocks = nbk * (nbk + 1) / 2;
-  const size_t adj_bytes = storage == KMT_ADJ_SYM ? sym_blocks * 256 * (256 + 4) * sizeof(R) : N * N * adj_elem;
+  const size_t adj_bytes = sym ? sym_blocks * 256 * (256 * adj_elem + 4 * sizeof(R)) : N * N * adj_elem;
   const size_t req = A ? 100 * N * sizeof(R) : (100 + 2 * K) * N * sizeof(R) + adj_bytes;
   std::printf("GPU Memory: %lu %lu %lu\n", (unsigned long)fr, (unsigned long)total, (unsigned long)req);
   std::fprintf(out, "GPU Memory: %lu %lu %lu\n", (unsigned long)fr, (unsigned long)total, (unsigned long)req);

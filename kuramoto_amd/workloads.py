@@ -30,6 +30,10 @@ PLAIN = {
                             t1=100.0, dt=1.0, storage="sym"),
     "fp32-n50000-sym": dict(N=50000, K=5, seed=2, prec="f32", normal=True, init=6.283185307, c1=3.0, atol=1e-6,
                             t1=100.0, dt=1.0, storage="sym"),
+    # the FP64 run with BF16 symmetric storage (2.5 GB): FP32 coupling sums,
+    # FP64 state, stage combinations and norm
+    "fp64-n50000-sym-bf16": dict(N=50000, K=5, seed=2, prec="f64", normal=True, init=6.283185307, c1=3.0,
+                                 atol=1e-9, t1=100.0, dt=1.0, storage="sym-bf16"),
     # the reference's default FP32 run: N=5000, Cauchy, theta0=0 (A fits the 256 MB cache)
     "fp32-n5000": dict(N=5000, K=5, seed=1, prec="f32", normal=False, init=0.0, c1=3.0, atol=1e-6, t1=100.0,
                        dt=1.0, storage="native"),

@@ -294,7 +294,8 @@ def test_cli_usage_exits_without_running(tmp_path, orc, prog, args):
     assert lines[0] == "usage:\tkuramoto [-hvnRFA] [-N N] [-K K] [-D D]"
     assert lines[-2] == ref_lines[-2] == "filebase is base file name for output. "
     assert lines[:-1] == ref_lines[:-1] and len(lines) == 24
-    assert lines[-1] == "extensions: --gauss --adj-dtype=native|bf16|f16|sym --rebuild --max-steps=n --chunk=n"
+    assert lines[-1] == ("extensions: --gauss --adj-dtype=native|bf16|f16|sym|sym-bf16|sym-f16 --rebuild "
+                         "--max-steps=n --chunk=n")
     assert os.listdir(tmp_path) == []
 
 
@@ -319,28 +320,31 @@ def test_cli_memory_check_stops_too_large_run(tmp_path, km, prog, N):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("prog_name", ["kuramoto", "kuramoto_64"])
-def test_cli_symmetric_storage(tmp_path, km, prog_name):
-    """`--adj-dtype=sym` (the upper block triangle only) runs the same
-    method on the same matrix: adj.dat written from the stored triangle is
-    byte-identical to the dense run's, and the run follows the dense one
-    (only the order of the additions differs): FP64 order.dat and phases
-    within 1e-9, FP32 order.dat within 1e-4, accepted steps within 2% (or
-    1).  A symmetric adj.dat reloads into symmetric storage (-R)."""
+@pytest.mark.parametrize("prog_name,sym_opt,dense_opt", [("kuramoto", "sym", "native"), ("kuramoto_64", "sym", "native"),
+                                                         ("kuramoto_64", "sym-bf16", "bf16")])
+def test_cli_symmetric_storage(tmp_path, km, prog_name, sym_opt, dense_opt):
+    """`--adj-dtype=sym` / `sym-bf16` (the upper block triangle only) runs
+    the same method on the same matrix as `native` / `bf16`: adj.dat
+    written from the stored triangle is byte-identical to the dense run's,
+    and the run follows the dense one (only the order of the additions
+    differs): FP64 order.dat and phases within 1e-9 (1e-5 with 16-bit
+    storage, whose sums are FP32), FP32 order.dat within 1e-4, accepted
+    steps within 2% (or 1).  A symmetric adj.dat reloads into symmetric
+    storage (-R)."""
     prog = os.path.join(BIN, prog_name)
     dt = np.float32 if prog_name == "kuramoto" else np.float64
     s, d = str(tmp_path / "s"), str(tmp_path / "d")
     args = ARGS + ["-D", "3", "-a", "1e-6" if prog_name == "kuramoto" else "1e-9"]
-    print(f"case cli {prog_name} --adj-dtype=sym", flush=True)
-    run(prog, args + ["--adj-dtype=sym"], s)
-    run(prog, args, d)
+    print(f"case cli {prog_name} --adj-dtype={sym_opt}", flush=True)
+    run(prog, args + [f"--adj-dtype={sym_opt}"], s)
+    run(prog, args + [f"--adj-dtype={dense_opt}"], d)
     check_layout(s, 120, N_OUT, dt, dense=3)
     for f in ("adj.dat", "frequencies.dat"):
         assert open(s + f, "rb").read() == open(d + f, "rb").read(), f
-    tol = 1e-9 if dt == np.float64 else 1e-4
+    tol = 1e-4 if dt == np.float32 else 1e-5 if sym_opt != "sym" else 1e-9
     assert np.max(np.abs(load(s, "order.dat", dt) - load(d, "order.dat", dt))) < tol
     if dt == np.float64:
-        assert np.max(np.abs(load(s, "thetas.dat", dt) - load(d, "thetas.dat", dt))) < 1e-9
+        assert np.max(np.abs(load(s, "thetas.dat", dt) - load(d, "thetas.dat", dt))) < tol
     ns, nd = load(s, "times.dat", dt).size, load(d, "times.dat", dt).size
     assert abs(ns - nd) <= max(1, int(0.02 * nd)), (ns, nd)
     # -R: the dense run's adjacency and frequencies, its initial phases as
@@ -349,7 +353,7 @@ def test_cli_symmetric_storage(tmp_path, km, prog_name):
     for f in ("adj.dat", "frequencies.dat"):
         shutil.copy(d + f, r + f)
     np.concatenate([load(d, "thetas.dat", dt)[:120], np.array([0.0, 0.02], dt)]).astype(dt).tofile(r + "fs.dat")
-    run(prog, args + ["-R", "--adj-dtype=sym"], r)
+    run(prog, args + ["-R", f"--adj-dtype={sym_opt}"], r)
     rr, rd = load(r, "order.dat", dt), load(d, "order.dat", dt)[1:]
     assert len(rr) == len(rd) and np.max(np.abs(rr - rd)) < 1e-3
     assert "Using adjacency from file" in open(r + ".out").read()

@@ -40,16 +40,18 @@ def _factor_case(km, prec, N, seed, K=5):
 
 
 @pytest.mark.parametrize("N", [1, 255, 256, 257, 600])
-@pytest.mark.parametrize("prec", ["f32", "f64"])
-def test_sym_storage_equals_dense(km, prec, N):
+@pytest.mark.parametrize("prec,storage", [("f32", ""), ("f64", ""), ("f32", "bf16"), ("f64", "bf16"), ("f64", "f16")])
+def test_sym_storage_equals_dense(km, prec, storage, N):
+    """Working precision and 16-bit storage (same power-of-two scale and
+    rounding as the dense 16-bit matrix)."""
     p = km.F32 if prec == "f32" else km.F64
     dtype, theta0, omega, X, w = _factor_case(km, p, N, 7)
-    print(f"case sym storage N={N} {prec}", flush=True)
+    print(f"case sym storage N={N} {prec} {storage}", flush=True)
     with km.Integrator(N, p, c1=3.0) as it:
         it.set_factor(X, w)
-        it.build_adjacency("native")
+        it.build_adjacency(storage or "native")
         dense = it.adjacency_rows(0, N)
-        it.build_adjacency("sym")
+        it.build_adjacency("sym-" + storage if storage else "sym")
         packed = it.adjacency_rows(0, N)
     assert np.array_equal(dense, packed)
     assert np.array_equal(packed, packed.T)
@@ -193,3 +195,53 @@ def test_sym_zero_rank_and_resume(km, prec):
     y1, t1, h1 = run(theta0, 0.0, 0.01, 4)
     y2, t2, h2 = run(y1, t1, h1, 5)
     assert t2 == t_full and h2 == h_full and np.array_equal(y2, y_full)
+
+
+@pytest.mark.parametrize("N", [257, 1000, 4097])
+@pytest.mark.parametrize("prec,storage", [("f32", "bf16"), ("f32", "f16"), ("f64", "bf16")])
+def test_sym16_rhs_matches_dense16(km, prec, storage, N):
+    """16-bit symmetric storage, packed from a host matrix: RHS and
+    couplings against the dense 16-bit path on the same matrix.  Both
+    accumulate in FP32 in different orders, each within 2e-6 of the row
+    scale of the exact sums, so they agree within 4e-6."""
+    dtype = np.float32 if prec == "f32" else np.float64
+    p = km.F32 if prec == "f32" else km.F64
+    rng = np.random.default_rng(23 + N)
+    M = rng.standard_normal((N, N)) / max(N, 1)
+    M = (0.5 * (M + M.T)).astype(dtype)
+    omega = rng.standard_normal(N).astype(dtype)
+    theta = (rng.uniform(-np.pi, np.pi, N) * 3).astype(dtype)
+    c1 = 1.75
+    print(f"case sym16 rhs N={N} {prec} {storage}", flush=True)
+    with km.Integrator(N, p, c1=c1) as it:
+        it.set_omega(omega)
+        it.set_adjacency(M, storage)
+        f_d, cp_d = it.rhs(theta, couplings=True)
+        Mq = it.adjacency_rows(0, N)
+        it.set_adjacency(M, "sym-" + storage)
+        assert np.array_equal(it.adjacency_rows(0, N), Mq)
+        f_s, cp_s = it.rhs(theta, couplings=True)
+    rs = np.abs(Mq.astype(np.float64)).sum(axis=1)
+    assert np.all(np.abs(f_s.astype(np.float64) - f_d) <= 4e-6 * (np.abs(omega.astype(np.float64)) + c1 * rs))
+    assert np.all(np.abs(cp_s.astype(np.float64) - cp_d) <= 4e-6 * np.repeat(rs, 2))
+
+
+def test_sym_bf16_fp64_integration_follows_dense_bf16(km):
+    """kuramoto_64 arithmetic with BF16 symmetric storage against the dense
+    BF16 storage of the same factor (FP64 state, FP32 coupling sums in
+    another order): r(t) within 1e-5 and accepted steps within 2% (or 1)."""
+    N, c1, t1 = 2000, 3.0, 1.0
+    dtype, theta0, omega, X, w = _factor_case(km, km.F64, N, 2)
+    te = _grid(t1, 0.1, dtype)
+    print("case sym-bf16 fp64 integration", flush=True)
+    out = {}
+    for storage in ("bf16", "sym-bf16"):
+        with km.Integrator(N, km.F64, atol=1e-9, c1=c1) as it:
+            it.set_omega(omega)
+            it.set_factor(X, w)
+            it.build_adjacency(storage)
+            it.set_state(theta0, 0.0, 0.01)
+            out[storage] = it.run(te, t1, keep_rows=False, ring_rows=8)
+    a, b = out["bf16"], out["sym-bf16"]
+    assert np.max(np.abs(a["r"] - b["r"])) < 1e-5
+    assert abs(a["status"].accepted - b["status"].accepted) <= max(1, int(0.02 * a["status"].accepted))
