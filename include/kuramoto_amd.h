@@ -41,8 +41,10 @@ extern "C" {
 
 /* adjacency storage in HBM */
 #define KMT_ADJ_NATIVE 0 /* same type as the working precision */
-#define KMT_ADJ_BF16 2   /* bfloat16 with FP32 accumulation (KMT_F32 only) */
-#define KMT_ADJ_F16 3    /* IEEE half with FP32 accumulation (KMT_F32 only) */
+#define KMT_ADJ_BF16 2   /* bfloat16 with FP32 accumulation of the sums (with
+                            KMT_F64 the state, stage combinations and error
+                            norm stay FP64) */
+#define KMT_ADJ_F16 3    /* IEEE half, the same way */
 #define KMT_ADJ_SYM 4    /* flag: only the upper block triangle of a symmetric
                             matrix (half the bytes per RHS), in the working
                             precision (KMT_ADJ_SYM) or in 16 bits
