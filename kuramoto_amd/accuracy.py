@@ -13,6 +13,7 @@ the same dequantized matrix (arithmetic error) and with FP64 on the exact
 generated matrix (arithmetic + storage error).
 
     python -m kuramoto_amd.accuracy plain fp32-n50000 [--storage bf16] [--t1 100]
+    python -m kuramoto_amd.accuracy storage64 [--t1 10]
     python -m kuramoto_amd.accuracy batch64 [--instances 0,9,18,...] [--t1 50]
     python -m kuramoto_amd.accuracy batch48 [--instances ...] [--t1 100]
 """
@@ -110,6 +111,39 @@ def plain(name, t1=None, storage=None):
     return out
 
 
+def storage64(t1=None):
+    """The FP64 N=50000 workload (kuramoto_64 arithmetic) with each adjacency
+    storage against the dense FP64 storage of the same factor: symmetric FP64
+    (the same sums in another order), dense and symmetric BF16 (FP32 sums of
+    BF16 entries; FP64 state, stage combinations and norm)."""
+    cfg = workloads.PLAIN["fp64-n50000"]
+    N, K, c1 = cfg["N"], cfg["K"], cfg["c1"]
+    t1 = t1 or 10.0
+    th, om = km.draws(km.F64, N, cfg["seed"], normal=cfg["normal"], init=cfg["init"])
+    X = km.factor(N, K, cfg["seed"], gaussian=True)
+    w = km.factor_weights(km.F64, N, K, gaussian=True)
+    te = _grid(t1, cfg["dt"]).astype(np.float64)
+    runs = {}
+    for storage in ("native", "sym", "bf16", "sym-bf16"):
+        with km.Integrator(N, km.F64, atol=cfg["atol"], c1=c1) as it:
+            it.set_omega(om)
+            it.set_factor(X, w)
+            it.build_adjacency(storage)
+            it.set_state(th, 0.0, 0.01)
+            res = it.run(te, t1, keep_rows=False, ring_rows=64)
+            y, _, _ = it.get_state()
+        runs[storage] = (res["r"], y, res["status"].accepted)
+        print(f"storage64 {storage}: {res['status'].accepted} accepted steps", flush=True)
+    r0, y0, a0 = runs["native"]
+    out = dict(config="fp64-n50000", t1=t1, accepted_dense_fp64=a0)
+    for k in ("sym", "bf16", "sym-bf16"):
+        r, y, acc = runs[k]
+        out[k] = dict(max_dr=float(np.max(np.abs(r - r0))),
+                      max_rel_phase=float(np.max(np.abs(y - y0) / np.maximum(1.0, np.abs(y0)))), accepted=acc)
+    out["sym-bf16_vs_bf16_max_dr"] = float(np.max(np.abs(runs["sym-bf16"][0] - runs["bf16"][0])))
+    return out
+
+
 def batch(which, instances, t1, ref_atol=1e-9, exact_only=False):
     name = "batch64-n20000-bf16" if which == "batch64" else "batch48-uneven-bf16"
     cfg = workloads.BATCH[name]
@@ -158,7 +192,7 @@ def batch(which, instances, t1, ref_atol=1e-9, exact_only=False):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("mode", choices=["plain", "batch64", "batch48"])
+    ap.add_argument("mode", choices=["plain", "storage64", "batch64", "batch48"])
     ap.add_argument("config", nargs="?", default="fp32-n50000")
     ap.add_argument("--instances", default=None, help="comma-separated instance indices (batch modes)")
     ap.add_argument("--t1", type=float, default=None)
@@ -168,6 +202,8 @@ def main():
     args = ap.parse_args()
     if args.mode == "plain":
         print(json.dumps(plain(args.config, args.t1, args.storage)), flush=True)
+    elif args.mode == "storage64":
+        print(json.dumps(storage64(args.t1)), flush=True)
     else:
         B = 64 if args.mode == "batch64" else 48
         if args.instances == "all":
