@@ -279,7 +279,17 @@ template <class R> struct Engine : EngineBase {
     const int el = stg & ~KMT_ADJ_SYM;
     if (stg < 0 || (el != KMT_ADJ_NATIVE && el != KMT_ADJ_BF16 && el != KMT_ADJ_F16))
       return fail(KMT_ERR_ARG, "unknown adjacency storage %d", stg);
-    if (is_sym(stg)) KMT_TRY(ensure_sym());
+    if (is_sym(stg)) {
+      KMT_TRY(ensure_sym());
+    } else if (sym_ij) {  // back to row-major storage: drop the block table and partials
+      drop_graphs();
+      HIP_TRY(hipStreamSynchronize(stream));
+      (void)hipFree(sym_ij);
+      (void)hipFree(sym_prow);
+      (void)hipFree(sym_pcol);
+      sym_ij = nullptr;
+      sym_prow = sym_pcol = nullptr;
+    }
     const size_t bytes = is_sym(stg) ? (size_t)sym_blocks * kSymT * kSymT * elem_bytes(stg)
                                      : (size_t)rows_alloc * (size_t)lda * elem_bytes(stg);
     if (A && A_bytes == bytes && storage == stg) return KMT_OK;

@@ -53,7 +53,9 @@ def test_sym_storage_equals_dense(km, prec, storage, N):
         dense = it.adjacency_rows(0, N)
         it.build_adjacency("sym-" + storage if storage else "sym")
         packed = it.adjacency_rows(0, N)
-    assert np.array_equal(dense, packed)
+        it.build_adjacency(storage or "native")  # and back: the block buffers are released
+        again = it.adjacency_rows(0, N)
+    assert np.array_equal(dense, packed) and np.array_equal(dense, again)
     assert np.array_equal(packed, packed.T)
 
 
