@@ -343,6 +343,8 @@ template <class R> struct Engine : EngineBase {
 
   int set_adjacency(const void *Mv, int src_prec, long ld, int stg) override {
     if (ld < N) return fail(KMT_ERR_ARG, "adjacency row stride %ld < N=%ld", ld, N);
+    HIP_TRY(hipStreamSynchronize(stream));
+    drop_graphs();  // captured launches hold the old matrix scale
     if (src_prec != (sizeof(R) == 4 ? KMT_F32 : KMT_F64))
       return fail(KMT_ERR_ARG, "adjacency source precision must match the context precision");
     const R *M = (const R *)Mv;
@@ -466,6 +468,8 @@ template <class R> struct Engine : EngineBase {
 
   int build_adjacency(int stg) override {
     if (!X) return fail(KMT_ERR_STATE, "kmt_build_adjacency needs kmt_set_factor first");
+    HIP_TRY(hipStreamSynchronize(stream));
+    drop_graphs();  // captured launches hold the old matrix scale
     KMT_TRY(ensure_A(stg));
     factored = 0;
     if (stg == KMT_ADJ_NATIVE) {
