@@ -357,3 +357,35 @@ def test_cli_symmetric_storage(tmp_path, km, prog_name, sym_opt, dense_opt):
     rr, rd = load(r, "order.dat", dt), load(d, "order.dat", dt)[1:]
     assert len(rr) == len(rd) and np.max(np.abs(rr - rd)) < 1e-3
     assert "Using adjacency from file" in open(r + ".out").read()
+
+
+@pytest.mark.gpu
+def test_cli_reference_quirks(tmp_path, km, orc):
+    """Quirks of the reference command line that the drop-in keeps:
+    Q3 (kuramoto.cu:294-339): "-nDv" reads "v" as the argument of -D, so the
+    dense level is 0 and the run is not verbose: the same files as
+    "-n -D 0", and the same set as the oracle program writes for "-nDv".
+    Q4 (kuramoto.cu:498-500): the N initial-phase draws are made even with
+    -I 0, so frequencies.dat is the same as with -I 6.28.
+    Q15 (kuramoto.cu:317): -g is parsed with atof, "-g 0.7" is device 0."""
+    prog = os.path.join(BIN, "kuramoto")
+    common = ["-N", "120", "-K", "5", "-c", "3", "-t", "1", "-d", "0.25", "-s", "3"]
+    p = {k: str(tmp_path / k) for k in ("q3", "d0", "oq3", "i0", "i6", "g07", "g0")}
+    print("case cli quirks Q3 Q4 Q15", flush=True)
+
+    def files(base):
+        d, stem = os.path.split(base)
+        return sorted(f[len(stem):] for f in os.listdir(d) if f.startswith(stem))
+
+    out_q3 = run(prog, common + ["-nDv"], p["q3"])
+    out_d0 = run(prog, common + ["-n", "-D", "0"], p["d0"])
+    run(os.path.join(ORC, "kuramoto_oracle"), common + ["-nDv"], p["oq3"])
+    assert files(p["q3"]) == files(p["d0"]) == files(p["oq3"])
+    assert "thetas.dat" not in files(p["q3"]) and len(out_q3.splitlines()) == len(out_d0.splitlines())
+    assert np.array_equal(load(p["q3"], "order.dat", np.float32), load(p["d0"], "order.dat", np.float32))
+    run(prog, common + ["-I", "0", "-D", "0"], p["i0"])
+    run(prog, common + ["-I", "6.283185307", "-D", "0"], p["i6"])
+    assert open(p["i0"] + "frequencies.dat", "rb").read() == open(p["i6"] + "frequencies.dat", "rb").read()
+    run(prog, common + ["-g", "0.7", "-D", "0"], p["g07"])
+    run(prog, common + ["-g", "0", "-D", "0"], p["g0"])
+    assert open(p["g07"] + "order.dat", "rb").read() == open(p["g0"] + "order.dat", "rb").read()
