@@ -389,3 +389,32 @@ def test_cli_reference_quirks(tmp_path, km, orc):
     run(prog, common + ["-g", "0.7", "-D", "0"], p["g07"])
     run(prog, common + ["-g", "0", "-D", "0"], p["g0"])
     assert open(p["g07"] + "order.dat", "rb").read() == open(p["g0"] + "order.dat", "rb").read()
+
+
+@pytest.mark.gpu
+def test_cli_reference_quirks_restart_and_overshoot(tmp_path, km, orc):
+    """Q5 (kuramoto.cu:475-489, 501-503): a FILEBASEfs.dat holding only N
+    values (what the startup write leaves) is not a restart file, so -R
+    falls back to the seed's random phases and the run equals the run
+    without -R (adjacency and frequencies regenerated from the seed too,
+    their files being absent).
+    Q6 (dp45.cu:153-155): the last step is not clamped, so the last
+    accepted time in times.dat reaches or passes t1, as in the oracle
+    program's run."""
+    prog = os.path.join(BIN, "kuramoto")
+    common = ["-N", "120", "-K", "5", "-c", "3", "-t", "1", "-d", "0.25", "-s", "3", "-I", "6.283185307"]
+    a, r, o = str(tmp_path / "a"), str(tmp_path / "r"), str(tmp_path / "o")
+    print("case cli quirks Q5 Q6", flush=True)
+    run(prog, common + ["-D", "1"], a)
+    np.random.default_rng(0).standard_normal(120).astype(np.float32).tofile(r + "fs.dat")
+    run(prog, common + ["-D", "1", "-R"], r)
+    assert open(a + "order.dat", "rb").read() == open(r + "order.dat", "rb").read()
+    assert open(a + "thetas.dat", "rb").read() == open(r + "thetas.dat", "rb").read()
+    ta = load(a, "times.dat", np.float32)
+    assert ta[-1] >= np.float32(1.0) and np.all(np.diff(ta) > 0)
+    run(os.path.join(ORC, "kuramoto_oracle"), common + ["-D", "1"], o)
+    to = load(o, "times.dat", np.float32)
+    # FP32 step sequences of two summation orders drift apart slightly (the
+    # CLI comparison above allows 2% in the times): both overshoot t1
+    assert to[-1] >= np.float32(1.0) and abs(len(ta) - len(to)) <= 1
+    assert abs(float(ta[-1]) - float(to[-1])) <= 0.02 * float(to[-1])
