@@ -367,7 +367,8 @@ def test_cli_reference_quirks(tmp_path, km, orc):
     "-n -D 0", and the same set as the oracle program writes for "-nDv".
     Q4 (kuramoto.cu:498-500): the N initial-phase draws are made even with
     -I 0, so frequencies.dat is the same as with -I 6.28.
-    Q15 (kuramoto.cu:317): -g is parsed with atof, "-g 0.7" is device 0."""
+    Q15 (kuramoto.cu:317): -g is parsed with atof, "-g 0.7" is device 0.
+    Q2: the default rank K is 6."""
     prog = os.path.join(BIN, "kuramoto")
     common = ["-N", "120", "-K", "5", "-c", "3", "-t", "1", "-d", "0.25", "-s", "3"]
     p = {k: str(tmp_path / k) for k in ("q3", "d0", "oq3", "i0", "i6", "g07", "g0")}
@@ -389,6 +390,10 @@ def test_cli_reference_quirks(tmp_path, km, orc):
     run(prog, common + ["-g", "0.7", "-D", "0"], p["g07"])
     run(prog, common + ["-g", "0", "-D", "0"], p["g0"])
     assert open(p["g07"] + "order.dat", "rb").read() == open(p["g0"] + "order.dat", "rb").read()
+    # Q2 (kuramoto.cu:278): without -K the rank is 6 (the usage text says 5)
+    k6 = str(tmp_path / "k6")
+    run(prog, [x for x in common if x not in ("-K", "5")] + ["-D", "0"], k6)
+    assert open(k6 + ".out").readline().split()[:2] == ["120", "6"]
 
 
 @pytest.mark.gpu
