@@ -103,6 +103,12 @@ def test_cli_matches_oracle_cli(tmp_path, km, orc, prec):
     assert np.max(np.abs(thg - tho) / np.maximum(1, np.abs(tho))) < tol
     fg, fo = load(g, "fs.dat", dt), load(o, "fs.dat", dt)
     assert np.max(np.abs(fg[:120] - fo[:120]) / np.maximum(1, np.abs(fo[:120]))) < tol
+    # Q7 (dp45.cu:160, kuramoto.cu:632-647): the final restart file holds the
+    # integrator's t (the last accepted time, past t1) and h, then couplings
+    assert fg[120] == tg[-1] and fo[120] == to[-1] and fg[120] >= dt(2.0)
+    if prec == "f64":
+        assert abs(fg[120] - fo[120]) < 1e-9 and abs(fg[121] - fo[121]) < 1e-6 * abs(fo[121])
+        assert np.max(np.abs(fg[122:] - fo[122:])) < 1e-9
     cg = load(g, "couplings.dat", dt).reshape(N_OUT, 240)
     co = load(o, "couplings.dat", dt).reshape(N_OUT, 240)
     assert np.max(np.abs(cg - co)) < (1e-9 if prec == "f64" else 1e-3)
