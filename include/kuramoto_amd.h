@@ -194,10 +194,14 @@ int kmt_bandwidth_probe(int device, size_t bytes, int reps, double *copy_gbps, d
  * per row group, balanced (one resident round, equal contiguous row
  * ranges)}, `reps` timed launches each; keeps the fastest.  variant_ms (12
  * values, may be NULL) receives the average ms of each variant, index
- * 4*r + 2*nt + balanced with r = 0, 1, 2 for 4, 8, 16 rows.  kmt_begin runs
- * it once by itself for N >= 4096.  The variants differ in speed only: every
- * row is summed in the same order.  KMT_ADJ_SYM has one form and is not
- * tuned.  No reference counterpart (cuBLAS chooses its own gemv kernel). */
+ * 4*r + 2*nt + balanced with r = 0, 1, 2 for 4, 8, 16 rows.  Every row is
+ * summed in the same order in every variant; the error norm's partial sums
+ * follow the variant's workgroups, so its last bits (and h's) can differ
+ * between variants.  Without this call kmt_begin uses a fixed choice (4 rows,
+ * plain loads when N >= 4096 and the matrix is <= 128 MB; else 8 rows,
+ * non-temporal), so that every context and process gives the same bits.  KMT_ADJ_SYM has one
+ * form and is not tuned.  No reference counterpart (cuBLAS chooses its own
+ * gemv kernel). */
 int kmt_autotune(kmt_ctx *ctx, int reps, double *variant_ms);
 /* Pin the dense stage-kernel variant instead of the autotuner's choice:
  * rows per group 4, 8 or 16; variant_flags as kmt_layout reports them. */

@@ -185,6 +185,26 @@ template <class R> struct Engine : EngineBase {
     if (flags) *flags = (tune_nt ? 1 : 0) | (tune_bal ? 2 : 0);
     return KMT_OK;
   }
+  // The stage-7 error partials are summed per workgroup, so their grouping,
+  // and with it the last bits of the error norm and of h, follows the
+  // variant.  kmt_begin therefore picks it by a fixed rule, not by timing,
+  // so that runs and restarts of one problem give the same bits in every
+  // context and process: from N = 4096 on, a matrix that fits the 256 MB
+  // Infinity Cache twice over takes 4 rows per group with plain loads, a
+  // larger one 8 rows with non-temporal loads (the autotuner's picks at
+  // N=5000 FP32 and N=50000 FP32/FP64); smaller N keep 8 rows, the grouping
+  // of the FP64 register-rebuild kernel.  kmt_autotune / kmt_select_variant
+  // override it.
+  void default_variant() {
+    const bool cached = N >= 4096 && (double)N * (double)N * (double)elem_bytes(storage) <= 128e6;
+    tune_rows = cached ? 4 : 8;
+    tune_nt = cached ? 0 : 1;
+    tune_bal = 0;
+    bal_grid = 0;
+    tuned = 1;
+    drop_graphs();
+  }
+
   int select_variant(int rows, int flags) override {
     if (rows != 4 && rows != 8 && rows != 16) return fail(KMT_ERR_ARG, "rows per group must be 4, 8 or 16 (got %d)", rows);
     if (flags & ~3) return fail(KMT_ERR_ARG, "unknown variant flags 0x%x", flags);
@@ -918,7 +938,7 @@ template <class R> struct Engine : EngineBase {
     shadow.times_cap = times_cap;
     shadow.rhs_evals = 1;
     KMT_TRY(push_state());
-    if (!tuned && !factored && N >= 4096) KMT_TRY(autotune(N >= 20000 ? 2 : 5));
+    if (!tuned && !factored && !sym()) default_variant();
     // k1 = f(y) (dp45.cu:151), then the first stage-2 vector
     sincos_kernel<R><<<(unsigned)((N + kBlock - 1) / kBlock), kBlock, 0, stream>>>(y, N, ldv, cs[1]);
     Vecs<R> v = vecs(cs[1], nullptr, 0);
