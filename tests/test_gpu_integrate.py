@@ -311,3 +311,35 @@ def test_fp64_state_with_16bit_adjacency(km, orc, storage):
     assert np.max(np.abs(g["r"] - ref["r"])) < 1e-6
     rel = np.abs(g["rows"][-1] - ref["rows"][-1]) / np.maximum(1.0, np.abs(ref["rows"][-1]))
     assert rel.max() < 1e-6, rel.max()
+
+
+@pytest.mark.parametrize("N,prec", [(5000, "f32"), (6000, "f64")])
+def test_restart_in_new_context_is_bit_exact(km, N, prec):
+    """12 attempts in one context equal 5 attempts, a new context started
+    from kmt_get_state (the FILEBASEfs.dat restart, kuramoto.cu:471-494) and
+    7 more, bit for bit.  Each context picks its stage-kernel variant in
+    kmt_begin by the same fixed rule (N=5000 FP32: 100 MB, 4 rows per
+    group; N=6000 FP64: 288 MB, 8 rows), so the error norm's partial sums
+    are grouped alike in every context."""
+    p = km.F32 if prec == "f32" else km.F64
+    dtype, theta0, omega, X, w = _inputs(km, p, N, 5, 4, normal=False, init=6.283185307)
+    te = np.arange(1, 101, dtype=np.float64)
+    atol = 1e-6 if prec == "f32" else 1e-9
+    print(f"case restart N={N} {prec}", flush=True)
+
+    def run(y0, t0, h0, attempts):
+        with km.Integrator(N, p, atol=atol, c1=3.0) as it:
+            it.set_omega(omega)
+            it.set_factor(X, w)
+            it.build_adjacency()
+            it.set_state(y0, t0, h0)
+            it.begin(te[te > t0], 100.0, keep_rows=False, ring_rows=64)
+            st = it.advance(attempts)
+            assert st.error == 0 and st.active == 1
+            return it.get_state() + (it.layout()[2],)
+
+    y_full, t_full, h_full, rows = run(theta0, 0.0, 0.01, 12)
+    y1, t1, h1, _ = run(theta0, 0.0, 0.01, 5)
+    y2, t2, h2, _ = run(y1, t1, h1, 7)
+    assert rows == (4 if prec == "f32" else 8)
+    assert t2 == t_full and h2 == h_full and np.array_equal(y2, y_full)
