@@ -324,13 +324,14 @@ def main():
             "frac": achieved / HBM_PEAK}
     if cfg["storage"] == "rebuild":
         # (K products + K fmas to form M_ij, 2 fmas into C and S) per pair:
-        # 3K + 4 flops in 2K + 2 VALU instructions per (i, j), against the
-        # published FP32 vector peak (157.3 TFLOP/s, all-FMA instruction mix)
+        # 3K + 4 flops in 2K + 3 VALU instructions per (i, j) (M_ij converted
+        # to double once; the sums accumulate in double, as the dense path's),
+        # against the published FP32 vector peak (157.3 TFLOP/s, all-FMA mix)
         flops = (3 * cfg["K"] + 4) * N * N
         tf = flops / (r["stage_ms"] / 1e3) / 1e12
         roof = {"bound": "valu", "achieved": tf, "peak": 157.3, "unit": "TFLOP/s", "frac": tf / 157.3,
                 "flops_per_launch": flops,
-                "valu_instructions_per_launch": (2 * cfg["K"] + 2) * N * N}
+                "valu_instructions_per_launch": (2 * cfg["K"] + 3) * N * N}
     out = {
         "metric": "osc*steps/s (N x accepted DP45 steps per second)",
         "value": N * accepted / secs,

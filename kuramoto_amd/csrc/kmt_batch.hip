@@ -91,7 +91,7 @@ __global__ __launch_bounds__(kBlock) void bsincos_kernel(const float *y, const B
   const long n = (long)blockIdx.x * kBlock + threadIdx.x;
   if (n >= inst[b].n) return;
   const float v = y[(size_t)b * ldn + n];
-  const float c = cosf(v), s = sinf(v);
+  const float c = rcos<float>(v), s = rsin<float>(v);
   cs[((size_t)b * 2) * ldn + n] = c;
   cs[((size_t)b * 2 + 1) * ldn + n] = s;
   if (write_vt) put_vt<T>(vt, inst[b].vrow, n, c, s);
@@ -368,6 +368,7 @@ __global__ __launch_bounds__(kBlock) void bgemv_kernel(const S *Abase, const siz
                                                        const GroupItem *items, const BInst *inst, const float *cs,
                                                        long ldn, float *P, int B, int check_state) {
   using Acc = typename Store<S>::Acc;
+  using Val = typename Store<S>::Val;
   constexpr int VEC = Store<S>::VEC;
   constexpr long STRIDE = (long)kBlock * VEC;
   const GroupItem it = items[blockIdx.x];
@@ -396,18 +397,18 @@ __global__ __launch_bounds__(kBlock) void bgemv_kernel(const S *Abase, const siz
     load_reals<float, VEC>(sinv + c + STRIDE, sv1);
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) {
-      Acc m[VEC];
+      Val m[VEC];
       unpack<S>(raw0[r], m);
 #pragma unroll
       for (int q = 0; q < VEC; ++q) {
-        accC[r] = rfma<Acc>(m[q], cv0[q], accC[r]);
-        accS[r] = rfma<Acc>(m[q], sv0[q], accS[r]);
+        accC[r] = rfma<Acc>((Acc)m[q], (Acc)cv0[q], accC[r]);
+        accS[r] = rfma<Acc>((Acc)m[q], (Acc)sv0[q], accS[r]);
       }
       unpack<S>(raw1[r], m);
 #pragma unroll
       for (int q = 0; q < VEC; ++q) {
-        accC[r] = rfma<Acc>(m[q], cv1[q], accC[r]);
-        accS[r] = rfma<Acc>(m[q], sv1[q], accS[r]);
+        accC[r] = rfma<Acc>((Acc)m[q], (Acc)cv1[q], accC[r]);
+        accS[r] = rfma<Acc>((Acc)m[q], (Acc)sv1[q], accS[r]);
       }
     }
   }
@@ -420,12 +421,12 @@ __global__ __launch_bounds__(kBlock) void bgemv_kernel(const S *Abase, const siz
     load_reals<float, VEC>(sinv + c, sv0);
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) {
-      Acc m[VEC];
+      Val m[VEC];
       unpack<S>(raw0[r], m);
 #pragma unroll
       for (int q = 0; q < VEC; ++q) {
-        accC[r] = rfma<Acc>(m[q], cv0[q], accC[r]);
-        accS[r] = rfma<Acc>(m[q], sv0[q], accS[r]);
+        accC[r] = rfma<Acc>((Acc)m[q], (Acc)cv0[q], accC[r]);
+        accS[r] = rfma<Acc>((Acc)m[q], (Acc)sv0[q], accS[r]);
       }
     }
   }
@@ -447,8 +448,8 @@ __global__ __launch_bounds__(kBlock) void bgemv_kernel(const S *Abase, const siz
       Acc C = red[0][tid], Sv = red[0][ROWS + tid];
 #pragma unroll
       for (int w = 1; w < kBlock / kWave; ++w) { C += red[w][tid]; Sv += red[w][ROWS + tid]; }
-      P[(size_t)it.b * ldn + i] = C;
-      P[(size_t)(B + it.b) * ldn + i] = Sv;
+      P[(size_t)it.b * ldn + i] = (float)C;
+      P[(size_t)(B + it.b) * ldn + i] = (float)Sv;
     }
   }
 }
@@ -510,7 +511,7 @@ __global__ __launch_bounds__(kBlock) void bepilogue_kernel(const BVecs a) {
       }
       const float yt = rfma<float>(acc, h, a.y[vb + n]);
       if constexpr (s + 1 == 7) a.ynew[vb + n] = yt;
-      const float c = cosf(yt), si = sinf(yt);
+      const float c = rcos<float>(yt), si = rsin<float>(yt);
       a.cs_out[2 * vb + n] = c;
       a.cs_out[2 * vb + a.ldn + n] = si;
       if (a.write_vt) put_vt<T>(a.vt, in->vrow, n, c, si);
@@ -707,7 +708,7 @@ __global__ __launch_bounds__(kBlock) void bemit_kernel(const BEmit a) {
   const float h = in->h;
   auto stage2 = [&](float yv, float k1) {
     const float yt = rfma<float>(k1 * (float)tab::A[0][0], h, yv);
-    const float c = cosf(yt), s = sinf(yt);
+    const float c = rcos<float>(yt), s = rsin<float>(yt);
     a.cs_out[2 * vb + n] = c;
     a.cs_out[2 * vb + a.ldn + n] = s;
     if (a.write_vt) put_vt<T>(a.vt, in->vrow, n, c, s);
@@ -735,8 +736,8 @@ __global__ __launch_bounds__(kBlock) void bemit_kernel(const BEmit a) {
       double cx = 0, sx = 0;
       if (valid) {
         const float val = dense_value<float>(Q, yl, hh, x);
-        cx = (double)cosf(val);
-        sx = (double)sinf(val);
+        cx = (double)rcos<float>(val);
+        sx = (double)rsin<float>(val);
       }
       const double bc = block_sum(cx, sh), bs = block_sum(sx, sh);
       if (threadIdx.x == 0) {

@@ -83,7 +83,7 @@ template <class R> struct Engine : EngineBase {
   // KMT_ADJ_SYM: (I, J) of each stored block, and the row / column partials
   long sym_nb = 0, sym_blocks = 0;
   int2 *sym_ij = nullptr;
-  SymPair<R> *sym_prow = nullptr, *sym_pcol = nullptr;
+  SymPair<double> *sym_prow = nullptr, *sym_pcol = nullptr;
   double *lr_part = nullptr, *lr_G = nullptr;
   int lr_nblk = 0, lr_grid = 0;
   std::vector<R> X_host, w_host;
@@ -286,11 +286,12 @@ template <class R> struct Engine : EngineBase {
       for (long J = I; J < sym_nb; ++J) ij[(size_t)sym_index(I, J, sym_nb)] = make_int2((int)I, (int)J);
     HIP_TRY(hipMalloc((void **)&sym_ij, sizeof(int2) * ij.size()));
     HIP_TRY(hipMemcpyAsync(sym_ij, ij.data(), sizeof(int2) * ij.size(), hipMemcpyHostToDevice, stream));
-    // sized for the wider of the two accumulation types (R, float)
-    HIP_TRY(hipMalloc((void **)&sym_prow, sizeof(SymPair<R>) * (size_t)sym_blocks * kSymT));
-    HIP_TRY(hipMalloc((void **)&sym_pcol, sizeof(SymPair<R>) * (size_t)sym_blocks * kSymT));
-    HIP_TRY(hipMemsetAsync(sym_prow, 0, sizeof(SymPair<R>) * (size_t)sym_blocks * kSymT, stream));
-    HIP_TRY(hipMemsetAsync(sym_pcol, 0, sizeof(SymPair<R>) * (size_t)sym_blocks * kSymT, stream));
+    // sized for the wider of the accumulation types (double for native
+    // storage, float for 16-bit storage)
+    HIP_TRY(hipMalloc((void **)&sym_prow, sizeof(SymPair<double>) * (size_t)sym_blocks * kSymT));
+    HIP_TRY(hipMalloc((void **)&sym_pcol, sizeof(SymPair<double>) * (size_t)sym_blocks * kSymT));
+    HIP_TRY(hipMemsetAsync(sym_prow, 0, sizeof(SymPair<double>) * (size_t)sym_blocks * kSymT, stream));
+    HIP_TRY(hipMemsetAsync(sym_pcol, 0, sizeof(SymPair<double>) * (size_t)sym_blocks * kSymT, stream));
     HIP_TRY(hipStreamSynchronize(stream));
     return KMT_OK;
   }

@@ -73,11 +73,17 @@ template <class R> struct State {
 struct bf16_t { uint16_t bits; };
 struct f16_t { uint16_t bits; };
 
+// VEC: stored values per 16-byte load; Val: the type a stored value unpacks
+// to; Acc: the type the coupling sums accumulate in.  Native FP32 storage
+// accumulates in double: the sums are then the correctly rounded float of
+// the exact sums up to the double rounding (the oracle rounds its double
+// sums the same way), so FP32 runs do not depend on the summation order.
+// 16-bit storage keeps FP32 accumulation.
 template <class S> struct Store;
-template <> struct Store<float> { static constexpr int VEC = 4; using Acc = float; };
-template <> struct Store<double> { static constexpr int VEC = 2; using Acc = double; };
-template <> struct Store<bf16_t> { static constexpr int VEC = 8; using Acc = float; };
-template <> struct Store<f16_t> { static constexpr int VEC = 8; using Acc = float; };
+template <> struct Store<float> { static constexpr int VEC = 4; using Val = float; using Acc = double; };
+template <> struct Store<double> { static constexpr int VEC = 2; using Val = double; using Acc = double; };
+template <> struct Store<bf16_t> { static constexpr int VEC = 8; using Val = float; using Acc = float; };
+template <> struct Store<f16_t> { static constexpr int VEC = 8; using Val = float; using Acc = float; };
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -91,8 +97,8 @@ __device__ __forceinline__ float bf16_hi(uint32_t w) { return __builtin_bit_cast
 __device__ __forceinline__ float f16_lo(uint32_t w) { return (float)__builtin_bit_cast(_Float16, (uint16_t)(w & 0xFFFFu)); }
 __device__ __forceinline__ float f16_hi(uint32_t w) { return (float)__builtin_bit_cast(_Float16, (uint16_t)(w >> 16)); }
 
-// 16 bytes of the adjacency -> VEC accumulator-typed values.
-template <class S> __device__ __forceinline__ void unpack(const u32x4 v, typename Store<S>::Acc *out);
+// 16 bytes of the adjacency -> VEC values.
+template <class S> __device__ __forceinline__ void unpack(const u32x4 v, typename Store<S>::Val *out);
 template <> __device__ __forceinline__ void unpack<float>(const u32x4 v, float *o) {
   const f32x4 f = __builtin_bit_cast(f32x4, v);
   o[0] = f.x; o[1] = f.y; o[2] = f.z; o[3] = f.w;
@@ -124,12 +130,18 @@ template <class R, int VEC> __device__ __forceinline__ void load_reals(const R *
   }
 }
 
-// Working-precision math (float -> the f-suffixed device functions).
+// Working-precision math.  FP32 cos / sin are the float roundings of the
+// double functions (within 1 ulp of the exact value), i.e. the correctly
+// rounded float except within 2^-29 relative of a rounding boundary: FP32
+// runs then do not depend on one float libm's last bit (the oracle rounds
+// the same way).  With Cauchy frequencies the fast oscillators' phases
+// reach 1e4, where one float ulp is 1e-3 rad, and a last-bit difference of
+// a cos or sin moves the whole population by ~1e-4 within t = 1.
 template <class R> __device__ __forceinline__ R rcos(R x) {
-  if constexpr (sizeof(R) == 4) return cosf(x); else return cos(x);
+  if constexpr (sizeof(R) == 4) return (float)cos((double)x); else return cos(x);
 }
 template <class R> __device__ __forceinline__ R rsin(R x) {
-  if constexpr (sizeof(R) == 4) return sinf(x); else return sin(x);
+  if constexpr (sizeof(R) == 4) return (float)sin((double)x); else return sin(x);
 }
 template <class R> __device__ __forceinline__ R rabs(R x) {
   if constexpr (sizeof(R) == 4) return fabsf(x); else return fabs(x);
@@ -321,6 +333,7 @@ __global__ __launch_bounds__(kBlock) void stage_kernel(const DenseArgs<R, S> a) 
     if (a.v.check_state && (!a.v.st->active || a.v.st->commit_pending)) return;
   }
   using Acc = typename Store<S>::Acc;
+  using Val = typename Store<S>::Val;
   constexpr int VEC = Store<S>::VEC;
   constexpr long STRIDE = (long)kBlock * VEC;
   const int tid = threadIdx.x;
@@ -356,28 +369,36 @@ __global__ __launch_bounds__(kBlock) void stage_kernel(const DenseArgs<R, S> a) 
     for (; c + STRIDE < a.lda; c += 2 * STRIDE) {
       R cv0[VEC], sv0[VEC], cv1[VEC], sv1[VEC];
       u32x4 raw0[ROWS], raw1[ROWS];
-#pragma unroll
-      for (int r = 0; r < ROWS; ++r) raw0[r] = load_row(rowp[r] + c);
-#pragma unroll
-      for (int r = 0; r < ROWS; ++r) raw1[r] = load_row(rowp[r] + c + STRIDE);
+      // issue order: the (cos, sin) chunks (L2 hits, needed by every row)
+      // first, then all 2 ROWS adjacency loads, then the first use.  The
+      // loads complete in issue order, so row r of the first chunk waits
+      // only for its own load.  Without the scheduling barriers the
+      // scheduler sinks the second chunk's loads next to their uses (one
+      // load in flight per wave) to save registers.
       load_reals<R, VEC>(cosv + c, cv0);
       load_reals<R, VEC>(sinv + c, sv0);
       load_reals<R, VEC>(cosv + c + STRIDE, cv1);
       load_reals<R, VEC>(sinv + c + STRIDE, sv1);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int r = 0; r < ROWS; ++r) raw0[r] = load_row(rowp[r] + c);
+#pragma unroll
+      for (int r = 0; r < ROWS; ++r) raw1[r] = load_row(rowp[r] + c + STRIDE);
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int r = 0; r < ROWS; ++r) {
-        Acc m[VEC];
+        Val m[VEC];
         unpack<S>(raw0[r], m);
 #pragma unroll
         for (int q = 0; q < VEC; ++q) {
-          accC[r] = rfma<Acc>(m[q], (Acc)cv0[q], accC[r]);
-          accS[r] = rfma<Acc>(m[q], (Acc)sv0[q], accS[r]);
+          accC[r] = rfma<Acc>((Acc)m[q], (Acc)cv0[q], accC[r]);
+          accS[r] = rfma<Acc>((Acc)m[q], (Acc)sv0[q], accS[r]);
         }
         unpack<S>(raw1[r], m);
 #pragma unroll
         for (int q = 0; q < VEC; ++q) {
-          accC[r] = rfma<Acc>(m[q], (Acc)cv1[q], accC[r]);
-          accS[r] = rfma<Acc>(m[q], (Acc)sv1[q], accS[r]);
+          accC[r] = rfma<Acc>((Acc)m[q], (Acc)cv1[q], accC[r]);
+          accS[r] = rfma<Acc>((Acc)m[q], (Acc)sv1[q], accS[r]);
         }
       }
     }
@@ -390,12 +411,12 @@ __global__ __launch_bounds__(kBlock) void stage_kernel(const DenseArgs<R, S> a) 
       load_reals<R, VEC>(sinv + c, sv0);
 #pragma unroll
       for (int r = 0; r < ROWS; ++r) {
-        Acc m[VEC];
+        Val m[VEC];
         unpack<S>(raw0[r], m);
 #pragma unroll
         for (int q = 0; q < VEC; ++q) {
-          accC[r] = rfma<Acc>(m[q], (Acc)cv0[q], accC[r]);
-          accS[r] = rfma<Acc>(m[q], (Acc)sv0[q], accS[r]);
+          accC[r] = rfma<Acc>((Acc)m[q], (Acc)cv0[q], accC[r]);
+          accS[r] = rfma<Acc>((Acc)m[q], (Acc)sv0[q], accS[r]);
         }
       }
     }
@@ -411,7 +432,9 @@ __global__ __launch_bounds__(kBlock) void stage_kernel(const DenseArgs<R, S> a) 
 // -A with register-rebuilt adjacency tiles: M_ij = sum_m w_m (X_im X_jm)
 // is formed in registers for every (row, column) pair of a stage from the
 // transposed factor XT (K x ldv, zero columns past N) — no N x N matrix in
-// HBM, (2K + 2) N^2 VALU operations per RHS.  Each entry uses the dense
+// HBM, (2K + 2) N^2 VALU operations per RHS (K products and K fmas per
+// entry, two fmas into the double sums; FP32 adds the entry's conversion to
+// double: 2K + 3).  Each entry uses the dense
 // generator's formula and the columns are visited in the dense stage
 // kernel's order (VEC per 16-byte chunk, chunks strided by kBlock VEC), so
 // the result equals the dense FP32/FP64 path on the generated matrix bit
@@ -449,7 +472,8 @@ __global__ __launch_bounds__(kBlock) void rebuild_stage_kernel(const RebuildArgs
   }
   const R *cosv = a.v.cs_in;
   const R *sinv = a.v.cs_in + ldv;
-  R accC[ROWS], accS[ROWS];
+  using Acc = typename Store<R>::Acc;  // the dense path's accumulation
+  Acc accC[ROWS], accS[ROWS];
 #pragma unroll
   for (int r = 0; r < ROWS; ++r) { accC[r] = 0; accS[r] = 0; }
   for (long c = (long)threadIdx.x * VEC; c < ldv; c += STRIDE) {
@@ -467,11 +491,11 @@ __global__ __launch_bounds__(kBlock) void rebuild_stage_kernel(const RebuildArgs
 #pragma unroll
         for (int m = 0; m < KT; ++m)
           if (m < K) mij = rfma<R>(wm[m], xi[r][m] * xj[m][q], mij);
-        accC[r] = rfma<R>(mij, cv[q], accC[r]);
-        accS[r] = rfma<R>(mij, sv[q], accS[r]);
+        accC[r] = rfma<Acc>((Acc)mij, (Acc)cv[q], accC[r]);
+        accS[r] = rfma<Acc>((Acc)mij, (Acc)sv[q], accS[r]);
       }
   }
-  finish_rows<R, R, ROWS, KIND>(accC, accS, row0, (R)1, a.v);
+  finish_rows<R, Acc, ROWS, KIND>(accC, accS, row0, (R)1, a.v);
 }
 
 // ---------------------------------------------------------------------

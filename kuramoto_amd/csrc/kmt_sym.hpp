@@ -39,8 +39,8 @@ template <class T> struct alignas(2 * sizeof(T)) SymPair {
   T c, s;
 };
 
-// R: working precision; S: stored element; Acc: accumulation (R for native
-// storage, float for 16-bit storage).
+// R: working precision; S: stored element; Acc: accumulation (double for
+// native storage, float for 16-bit storage, as in the dense path).
 template <class R, class S> struct SymArgs {
   using Acc = typename Store<S>::Acc;
   const S *A;          // stored blocks
@@ -61,7 +61,7 @@ template <class S> struct SymLoad {
   static constexpr int VEC = BYTES / (int)sizeof(S);
   using Raw = typename std::conditional<BYTES == 16, u32x4, u32x2>::type;
 };
-template <class S> __device__ __forceinline__ void sym_unpack(const u32x4 v, typename Store<S>::Acc *o) { unpack<S>(v, o); }
+template <class S> __device__ __forceinline__ void sym_unpack(const u32x4 v, typename Store<S>::Val *o) { unpack<S>(v, o); }
 template <class S> __device__ __forceinline__ void sym_unpack(const u32x2 v, float *o);
 template <> __device__ __forceinline__ void sym_unpack<bf16_t>(const u32x2 v, float *o) {
   o[0] = bf16_lo(v.x); o[1] = bf16_hi(v.x); o[2] = bf16_lo(v.y); o[3] = bf16_hi(v.y);
@@ -134,6 +134,7 @@ template <class R, class S>
 __global__ __launch_bounds__(kBlock) void sym_tile_kernel(const SymArgs<R, S> a) {
   if (a.v.check_state && (!a.v.st->active || a.v.st->commit_pending)) return;
   using Acc = typename Store<S>::Acc;
+  using Val = typename Store<S>::Val;
   using Raw = typename SymLoad<S>::Raw;
   constexpr int VEC = SymLoad<S>::VEC;
   constexpr int NLD = kSymT / (kWave * VEC);
@@ -187,9 +188,12 @@ __global__ __launch_bounds__(kBlock) void sym_tile_kernel(const SymArgs<R, S> a)
     Acc part[2 * RB];
 #pragma unroll
     for (int r = 0; r < RB; ++r) {
+      Val mv[NC];
+#pragma unroll
+      for (int q = 0; q < NLD; ++q) sym_unpack<S>(raw[r][q], mv + q * VEC);
       Acc m[NC];
 #pragma unroll
-      for (int q = 0; q < NLD; ++q) sym_unpack<S>(raw[r][q], m + q * VEC);
+      for (int k = 0; k < NC; ++k) m[k] = (Acc)mv[k];
       const int row = row_of(kb + r);
       const Acc ci = vi[0][row], si = vi[1][row];
       Acc pc = m[0] * cj[0], ps = m[0] * sj[0];

@@ -20,7 +20,13 @@
  *                        build evaluates the quartic in Horner form.
  *   rhs_*                kuramoto.cu:42-55, :104-126; pinned by closed forms
  *                        (M=0, Adler N=2, symmetric-M mean-frequency
- *                        invariant) and a brute-force O(N^2) loop.
+ *                        invariant) and a brute-force O(N^2) loop.  Coupling
+ *                        sums accumulate in double and round once to R;
+ *                        FP32 cos / sin are the float roundings of the
+ *                        double functions (ocos / osin).  The reference's
+ *                        cuBLAS summation order and CUDA cosf bits are not
+ *                        reproducible (no CUDA here), so FP32 parity rests
+ *                        on these two roundings, which the GPU follows.
  *   frequencies / ICs    kuramoto.cu:495-541; pinned bit-exactly by
  *                        data/5000{,_lorentz}/{0,2,5000}/1frequencies.dat.
  *   factor generators    the build's own Philox4x32-10 mapping (the
@@ -86,6 +92,16 @@ template <class R> inline Tableau<R> tableau() {
 
 inline float fmaR(float a, float b, float c) { return std::fma(a, b, c); }
 inline double fmaR(double a, double b, double c) { return std::fma(a, b, c); }
+
+/* cos / sin in the working precision (kuramoto.cu:52-53 evaluates them in
+ * the working precision).  FP32: the float rounding of the double function,
+ * i.e. the correctly rounded float value except within 2^-29 relative of a
+ * rounding boundary, so the restatement does not depend on one float libm's
+ * last bit (CUDA's cosf, glibc's cosf and the device library each differ
+ * from it in some last bits).  Pinned against numpy's float64 cos / sin
+ * rounded to float32 (tests/test_oracle.py). */
+template <class R> inline R ocos(R x) { return (R)std::cos((double)x); }
+template <class R> inline R osin(R x) { return (R)std::sin((double)x); }
 
 /* ------------------------------------------------------------------ */
 /* Host generators: frequencies and initial conditions from glibc rand */
@@ -216,8 +232,8 @@ template <class R> struct Model {
   void sincos(const R *y) {
 #pragma omp parallel for schedule(static)
     for (long i = 0; i < N; ++i) {
-      cosv[i] = std::cos(y[i]);
-      sinv[i] = std::sin(y[i]);
+      cosv[i] = ocos<R>(y[i]);
+      sinv[i] = osin<R>(y[i]);
     }
   }
 
@@ -259,27 +275,37 @@ template <class R> struct Model {
           const R *X = (const R *)adj.X, *w = (const R *)adj.w;
           int K = adj.K;
           for (long j = 0; j < N; ++j) row[j] = adj_entry<R>(X + i * K, X + j * K, w, K);
-          for (long j = 0; j < N; ++j) {
-            ac += (double)row[j] * (double)cosv[j];
-            as += (double)row[j] * (double)sinv[j];
-          }
+          row_sums(row.data(), ac, as);
         } else if (adj.Mf) {
-          const float *Mi = adj.Mf + (size_t)i * N;
-          for (long j = 0; j < N; ++j) {
-            ac += (double)Mi[j] * (double)cosv[j];
-            as += (double)Mi[j] * (double)sinv[j];
-          }
+          row_sums(adj.Mf + (size_t)i * N, ac, as);
         } else {
-          const double *Mi = adj.Md + (size_t)i * N;
-          for (long j = 0; j < N; ++j) {
-            ac += Mi[j] * (double)cosv[j];
-            as += Mi[j] * (double)sinv[j];
-          }
+          row_sums(adj.Md + (size_t)i * N, ac, as);
         }
         Cs[2 * i] = (R)(adj.mscale * ac);
         Cs[2 * i + 1] = (R)(adj.mscale * as);
       }
     }
+  }
+
+  /* C = sum_j m_j cos_j, S = sum_j m_j sin_j in double: four interleaved
+   * partial sums (j mod 4), combined as (p0 + p1) + (p2 + p3).  Any double
+   * order rounds to the same float sums except at a rounding boundary; four
+   * independent chains let the loop run at memory speed instead of the
+   * latency of one chain of additions. */
+  template <class T> void row_sums(const T *m, double &C, double &S) const {
+    double pc[4] = {0, 0, 0, 0}, ps[4] = {0, 0, 0, 0};
+    long j = 0;
+    for (; j + 4 <= N; j += 4)
+      for (int q = 0; q < 4; ++q) {
+        pc[q] += (double)m[j + q] * (double)cosv[j + q];
+        ps[q] += (double)m[j + q] * (double)sinv[j + q];
+      }
+    for (int q = 0; j < N; ++j, ++q) {
+      pc[q] += (double)m[j] * (double)cosv[j];
+      ps[q] += (double)m[j] * (double)sinv[j];
+    }
+    C = (pc[0] + pc[1]) + (pc[2] + pc[3]);
+    S = (ps[0] + ps[1]) + (ps[2] + ps[3]);
   }
 
   /* dydt (kuramoto.cu:118-126) */
@@ -295,8 +321,8 @@ template <class R> struct Model {
   R order(const R *y) {
     double X = 0, Y = 0;
     for (long i = 0; i < N; ++i) {
-      X += (double)std::cos(y[i]);
-      Y += (double)std::sin(y[i]);
+      X += (double)ocos<R>(y[i]);
+      Y += (double)osin<R>(y[i]);
     }
     R XR = (R)X, YR = (R)Y, NR = (R)N;
     return (R)std::pow((double)(XR / NR * XR / NR + YR / NR * YR / NR), 0.5);

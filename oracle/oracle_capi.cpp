@@ -45,6 +45,14 @@ void orc_draws_f64(long N, int seed, int normal, double freq, double I, double *
   for (long j = 0; j < N; ++j) omega[j] = freq_draw<double>(freq, normal);
 }
 
+/* The FP32 / FP64 cos and sin the model uses (ocos / osin). */
+void orc_sincos_f32(long N, const float *x, float *c, float *s) {
+  for (long i = 0; i < N; ++i) { c[i] = ocos<float>(x[i]); s[i] = osin<float>(x[i]); }
+}
+void orc_sincos_f64(long N, const double *x, double *c, double *s) {
+  for (long i = 0; i < N; ++i) { c[i] = ocos<double>(x[i]); s[i] = osin<double>(x[i]); }
+}
+
 void orc_tableau_f64(double *a36, double *c6, double *e7, double *p28) {
   Tableau<double> T = tableau<double>();
   std::memcpy(a36, T.a, sizeof(T.a));

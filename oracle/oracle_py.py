@@ -90,6 +90,16 @@ def draws(N, seed, normal, freq=1.0, init=0.0, dtype=np.float32):
     return ic, om
 
 
+def sincos(x, dtype=np.float32):
+    """(cos, sin) of x in the working precision, as the oracle's model uses them."""
+    x = np.ascontiguousarray(x, dtype)
+    c = np.zeros_like(x)
+    s = np.zeros_like(x)
+    fn = load().orc_sincos_f32 if np.dtype(dtype) == np.float32 else load().orc_sincos_f64
+    fn(ctypes.c_long(x.size), _p(x), _p(c), _p(s))
+    return c, s
+
+
 def tableau(dtype=np.float64):
     a = np.zeros(36, dtype)
     c = np.zeros(6, dtype)

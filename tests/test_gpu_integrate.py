@@ -5,11 +5,13 @@ Reference: dp45_step / dp45_run / dp45_eval (dp45.cu:87-164), step_eval's
 output grid and order parameter (kuramoto.cu:164-249), main's final-state
 interpolation (kuramoto.cu:630-647).
 
-Tolerances: FP64 phases at t=1 within 1e-10 of the FP64 oracle on the same
-inputs; FP32 phases held to the accuracy of the reference's own FP32 method
-(see _fp32_ok: at atol=1e-6 the FP32 oracle is itself 5e-6..2e-4 from FP64,
-so a fixed 1e-5 bound is out of reach of the method, not of the kernels);
-r(t) within 1e-3; accepted-step counts within 2% (or 1).  The
+Tolerances: phases at t=1 within 1e-10 (FP64) / 1e-5 (FP32) of the oracle
+in the same precision on the same inputs, relative to max(1, |theta|);
+r(t) within 1e-3; accepted-step counts within 2% (or 1).  The FP32 bound
+holds because both sides accumulate the coupling sums in double and round
+cos / sin once from double, so they see the same FP32 values; with Cauchy
+frequencies a one-ulp difference in any of them moves the population by
+~1e-4 within t = 1, so the bound is also the check of that rounding.  The
 K=0 controller KAT is bit-exact in t and step count.
 """
 import json
@@ -71,25 +73,21 @@ def _inputs(km, prec, N, K, seed, normal, init, gauss=True):
     return dtype, theta0, omega, X, w
 
 
-def _fp32_ok(gpu, orc32, truth, floor=1e-5):
-    """FP32 acceptance: the reference's own FP32 method (the FP32 oracle)
-    is 5e-6..2e-4 away from FP64 at t=1 with atol=1e-6 (controller
-    truncation error and FP32 rounding of large Cauchy phases), so the GPU
-    FP32 result is held to the FP32 oracle's accuracy: its max relative
-    error against an FP64 truth run (atol 1e-12) may not exceed twice the
-    FP32 oracle's error plus 1e-5."""
-    scale = np.maximum(1.0, np.abs(truth))
-    e_gpu = np.max(np.abs(gpu.astype(np.float64) - truth) / scale)
-    e_orc = np.max(np.abs(orc32.astype(np.float64) - truth) / scale)
-    return e_gpu <= 2 * e_orc + floor, (e_gpu, e_orc)
+def _phase_err(a, b):
+    """max |a - b| / max(1, |b|)."""
+    b64 = np.asarray(b, np.float64)
+    return float(np.max(np.abs(np.asarray(a, np.float64) - b64) / np.maximum(1.0, np.abs(b64))))
+
+
+PHASE_TOL = {"f32": 1e-5, "f64": 1e-10}
 
 
 @pytest.mark.parametrize("prec,atol", [("f64", 1e-9), ("f32", 1e-6)])
 @pytest.mark.parametrize("N", [333, 2000])
 def test_dense_integration_matches_oracle(km, orc, prec, atol, N):
-    """FP64: phases at t=1 within 1e-10 of the FP64 oracle on the same
-    inputs.  FP32: see _fp32_ok.  Both: r(t) within 1e-3, accepted steps
-    within 2% (or 1)."""
+    """Phases at t=1 (and the final kmt_eval interpolation) within 1e-10
+    (FP64) / 1e-5 (FP32) of the oracle in the same precision on the same
+    inputs; r(t) within 1e-3, accepted steps within 2% (or 1)."""
     p = km.F32 if prec == "f32" else km.F64
     K, seed, c1, t1 = 5, 2, 3.0, 1.0
     dtype, theta0, omega, X, w = _inputs(km, p, N, K, seed, normal=True, init=6.283185307)
@@ -109,16 +107,10 @@ def test_dense_integration_matches_oracle(km, orc, prec, atol, N):
     assert g["rows"].shape == ref["rows"].shape
     assert np.max(np.abs(g["r"] - ref["r"])) < 1e-3
     assert len(g["times"]) == st.accepted
-    if prec == "f64":
-        rel = np.abs(g["rows"] - ref["rows"]) / np.maximum(1.0, np.abs(ref["rows"]))
-        assert rel[-1].max() < 1e-10  # phases at t = 1
-        assert np.max(np.abs(fin - ref["final"]) / np.maximum(1.0, np.abs(ref["final"]))) < 1e-10
-    else:
-        M64 = orc.adj_rows(X.astype(np.float64), w.astype(np.float64), 0, N)
-        truth = orc.integrate(orc.Adj(0, M=M64), omega.astype(np.float64), c1, theta0.astype(np.float64), 0.0,
-                              0.01, t1, 1e-12, t_eval=te.astype(np.float64))
-        ok, errs = _fp32_ok(g["rows"][-1], ref["rows"][-1], truth["rows"][-1])
-        assert ok, errs
+    e1 = _phase_err(g["rows"][-1], ref["rows"][-1])  # phases at t = 1
+    ef = _phase_err(fin, ref["final"])
+    print(f"  accepted gpu={st.accepted} oracle={ref['accepted']} phase err t=1 {e1:.3e} final {ef:.3e}", flush=True)
+    assert e1 < PHASE_TOL[prec] and ef < PHASE_TOL[prec], (e1, ef)
 
 
 def test_ring_stall_matches_large_ring(km):
@@ -160,16 +152,21 @@ def test_fixed_step_matches_oracle(km, orc):
 @pytest.mark.parametrize("prec", ["f32", "f64"])
 @pytest.mark.parametrize("K", [5, 12, 20])
 def test_factored_integration_matches_dense(km, orc, prec, K):
-    """-A (factored) and the dense matrix built from the same factor follow
-    the same trajectory (reference: kuramoto vs kuramoto -A), checked
-    against the factored oracle (FP64: 1e-10; FP32: _fp32_ok).  K = 5 and
-    12 run the one-launch-per-stage low-rank kernels (rank tiles of 8 and
-    16), K = 20 the three-launch path for larger ranks."""
+    """-A (factored) and the dense matrix built from the same factor
+    (reference: kuramoto vs kuramoto -A).  Each GPU run against the oracle
+    run of the same form: phases at t=1 within 1e-10 (FP64) / 1e-5 (FP32),
+    r(t) within 1e-3, accepted steps within 2% (or 1); and the two forms
+    against each other: r(t) within 1e-3, steps within 2% (or 1).  (In
+    FP32 the two forms differ by the rounding of the stored M_ij, which
+    with Cauchy frequencies moves the phases by ~2e-4 at t=1 in the oracle
+    too.)  K = 5 and 12 run the one-launch-per-stage low-rank kernels (rank
+    tiles of 8 and 16), K = 20 the three-launch path for larger ranks."""
     p = km.F32 if prec == "f32" else km.F64
     N, seed, c1, t1 = 1500, 3, 3.0, 1.0
     atol = 1e-6 if prec == "f32" else 1e-9
     dtype, theta0, omega, X, w = _inputs(km, p, N, K, seed, normal=False, init=0.0)
     te = _grid(0.0, t1, 0.1, dtype)
+    print(f"case factored K={K} prec={prec}", flush=True)
     res = {}
     for mode in ("dense", "factored"):
         with km.Integrator(N, p, atol=atol, c1=c1) as it:
@@ -181,19 +178,21 @@ def test_factored_integration_matches_dense(km, orc, prec, K):
                 it.use_factored(True)
             it.set_state(theta0, 0.0, 0.01)
             res[mode] = it.run(te, t1, keep_rows=True)
-    ref = orc.integrate(orc.Adj(2, X=X, w=w), omega, c1, theta0, 0.0, 0.01, t1, atol, t_eval=te, dtype=dtype)
-    if prec == "f32":
-        truth = orc.integrate(orc.Adj(2, X=X.astype(np.float64), w=w.astype(np.float64)), omega.astype(np.float64),
-                              c1, theta0.astype(np.float64), 0.0, 0.01, t1, 1e-12, t_eval=te.astype(np.float64))
+    refs = {"dense": orc.integrate(orc.Adj(0, M=orc.adj_rows(X, w, 0, N)), omega, c1, theta0, 0.0, 0.01, t1, atol,
+                                   t_eval=te, dtype=dtype),
+            "factored": orc.integrate(orc.Adj(2, X=X, w=w), omega, c1, theta0, 0.0, 0.01, t1, atol, t_eval=te,
+                                      dtype=dtype)}
     for mode in res:
-        assert abs(res[mode]["status"].accepted - ref["accepted"]) <= max(1, int(0.02 * ref["accepted"]))
-        assert np.max(np.abs(res[mode]["r"] - ref["r"])) < 1e-3
-        if prec == "f64":
-            rel = np.abs(res[mode]["rows"][-1] - ref["rows"][-1]) / np.maximum(1.0, np.abs(ref["rows"][-1]))
-            assert rel.max() < 1e-10, (mode, rel.max())
-        else:
-            ok, errs = _fp32_ok(res[mode]["rows"][-1], ref["rows"][-1], truth["rows"][-1])
-            assert ok, (mode, errs)
+        g, ref = res[mode], refs[mode]
+        e1 = _phase_err(g["rows"][-1], ref["rows"][-1])
+        print(f"  {mode}: accepted gpu={g['status'].accepted} oracle={ref['accepted']} phase err t=1 {e1:.3e}",
+              flush=True)
+        assert abs(g["status"].accepted - ref["accepted"]) <= max(1, int(0.02 * ref["accepted"]))
+        assert np.max(np.abs(g["r"] - ref["r"])) < 1e-3
+        assert e1 < PHASE_TOL[prec], (mode, e1)
+    d, f = res["dense"], res["factored"]
+    assert abs(d["status"].accepted - f["status"].accepted) <= max(1, int(0.02 * d["status"].accepted))
+    assert np.max(np.abs(d["r"] - f["r"])) < 1e-3
 
 
 def test_resume_is_bit_exact(km):

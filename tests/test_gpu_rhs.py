@@ -5,9 +5,12 @@ Reference: dydt / makecoupling / tot_kuramoto / makey2 (kuramoto.cu:42-55,
 :104-126) and getadj (kuramoto.cu:145-161).
 
 Tolerances (stated per test): FP64 sums agree to 1e-12 of the row scale
-sum_j |M_ij| (different summation order only); FP32 sums accumulate in float
-on the GPU and in double in the oracle, so they agree to 2e-6 of the row
-scale; the adjacency generator is bit-exact (same fma chain on both sides).
+sum_j |M_ij| (different summation order only); FP32 sums accumulate in
+double on both sides and round once to float, so they are bit-identical
+except where the two double sums straddle a float rounding boundary (at
+most one float ulp, 1.2e-7 of the row scale; at least 99% of the values
+bit-identical); 16-bit storage accumulates in FP32 (2e-6 of the row scale);
+the adjacency generator is bit-exact (same fma chain on both sides).
 """
 import numpy as np
 import pytest
@@ -44,11 +47,14 @@ def test_rhs_dense_matches_oracle(km, orc, N, prec):
         it.set_adjacency(M)
         f, cp = it.rhs(theta, couplings=True)
     fo, cso = orc.rhs(orc.Adj(0, M=M), omega, c1, theta, dtype=dtype)
-    tol = (2e-6 if prec == "f32" else 1e-12) * _scale(M, omega, c1)
+    tol = (2.5e-7 if prec == "f32" else 1e-12) * _scale(M, omega, c1)
     assert np.all(np.abs(f.astype(np.float64) - fo) <= tol + 1e-30)
     rs = np.abs(M.astype(np.float64)).sum(axis=1)
-    tolc = (2e-6 if prec == "f32" else 1e-12) * np.repeat(rs, 2)
+    tolc = (1.2e-7 if prec == "f32" else 1e-12) * np.repeat(rs, 2)
     assert np.all(np.abs(cp.astype(np.float64) - cso) <= tolc + 1e-30)
+    if prec == "f32":  # double accumulation rounded once: the same floats
+        assert np.count_nonzero(cp == cso) >= 0.99 * cp.size
+        assert np.count_nonzero(f == fo) >= 0.99 * f.size
 
 
 def test_rhs_zero_adjacency_is_omega(km):
@@ -149,8 +155,10 @@ def test_factored_rhs_matches_oracle(km, orc, K, prec):
         it.use_factored(True)
         f = it.rhs(theta)
     fo, _ = orc.rhs(orc.Adj(2, X=X, w=w), omega, 1.75, theta, dtype=dtype)
-    tol = (1e-6 if prec == "f32" else 1e-13) * (np.abs(omega) + 1.75 * K)
+    tol = (2.5e-7 if prec == "f32" else 1e-13) * (np.abs(omega) + 1.75 * K)
     assert np.all(np.abs(f.astype(np.float64) - fo) <= tol + 1e-12 * (prec == "f32"))
+    if prec == "f32":  # G and C, S in double on both sides, rounded once
+        assert np.count_nonzero(f == fo) >= 0.99 * N
 
 
 def test_order_parameter(km):

@@ -10,7 +10,8 @@ its two gemvs (kuramoto.cu:104-126); only the summation order differs.
 * A symmetric host matrix packed by kmt_set_adjacency reads back unchanged;
   a non-symmetric one is refused (KMT_ERR_ARG).
 * RHS and couplings against the oracle, with the dense path's tolerances
-  (FP64 1e-12, FP32 2e-6 of the row scale sum_j |M_ij|).
+  (FP64 1e-12 of the row scale sum_j |M_ij|; FP32 sums accumulate in double
+  and round once, so within one float ulp, 1.2e-7 of the row scale).
 * Whole integrations against the oracle, with the dense path's tolerances.
   Two runs agree bit for bit.
 * At N = 50000 in FP64 (10 GB stored instead of 20 GB), the RHS equals the
@@ -91,19 +92,20 @@ def test_sym_rhs_matches_oracle(km, orc, N, prec):
         f, cp = it.rhs(theta, couplings=True)
     fo, cso = orc.rhs(orc.Adj(0, M=M), omega, c1, theta, dtype=dtype)
     rs = np.abs(M.astype(np.float64)).sum(axis=1)
-    tol = (2e-6 if prec == "f32" else 1e-12) * (np.abs(omega.astype(np.float64)) + c1 * rs)
+    tol = (2.5e-7 if prec == "f32" else 1e-12) * (np.abs(omega.astype(np.float64)) + c1 * rs)
     assert np.all(np.abs(f.astype(np.float64) - fo) <= tol + 1e-30)
-    tolc = (2e-6 if prec == "f32" else 1e-12) * np.repeat(rs, 2)
+    tolc = (1.2e-7 if prec == "f32" else 1e-12) * np.repeat(rs, 2)
     assert np.all(np.abs(cp.astype(np.float64) - cso) <= tolc + 1e-30)
+    if prec == "f32":
+        assert np.count_nonzero(cp == cso) >= 0.99 * cp.size
 
 
 @pytest.mark.parametrize("prec,atol", [("f64", 1e-9), ("f32", 1e-6)])
 @pytest.mark.parametrize("N", [333, 2000])
 def test_sym_integration_matches_oracle(km, orc, prec, atol, N):
     """Same acceptance as the dense path: accepted steps within 2% (or 1),
-    r(t) within 1e-3; FP64 phases at t=1 within 1e-10 of the FP64 oracle;
-    FP32 phases no further from an FP64 truth run (atol 1e-12) than twice
-    the FP32 oracle's distance + 1e-5."""
+    r(t) within 1e-3; phases at t=1 within 1e-10 (FP64) / 1e-5 (FP32) of
+    the oracle in the same precision (both accumulate the sums in double)."""
     p = km.F32 if prec == "f32" else km.F64
     c1, t1 = 3.0, 1.0
     dtype, theta0, omega, X, w = _factor_case(km, p, N, 2)
@@ -124,17 +126,10 @@ def test_sym_integration_matches_oracle(km, orc, prec, atol, N):
     st = g["status"]
     assert abs(st.accepted - ref["accepted"]) <= max(1, int(0.02 * ref["accepted"]))
     assert np.max(np.abs(g["r"] - ref["r"])) < 1e-3
-    if prec == "f64":
-        rel = np.abs(g["rows"][-1] - ref["rows"][-1]) / np.maximum(1.0, np.abs(ref["rows"][-1]))
-        assert rel.max() < 1e-10, rel.max()
-    else:
-        M64 = orc.adj_rows(X.astype(np.float64), w.astype(np.float64), 0, N)
-        truth = orc.integrate(orc.Adj(0, M=M64), omega.astype(np.float64), c1, theta0.astype(np.float64), 0.0,
-                              0.01, t1, 1e-12, t_eval=te.astype(np.float64))["rows"][-1]
-        scale = np.maximum(1.0, np.abs(truth))
-        e_gpu = np.max(np.abs(g["rows"][-1].astype(np.float64) - truth) / scale)
-        e_orc = np.max(np.abs(ref["rows"][-1].astype(np.float64) - truth) / scale)
-        assert e_gpu <= 2 * e_orc + 1e-5, (e_gpu, e_orc)
+    ref1 = ref["rows"][-1].astype(np.float64)
+    rel = np.abs(g["rows"][-1].astype(np.float64) - ref1) / np.maximum(1.0, np.abs(ref1))
+    print(f"  accepted gpu={st.accepted} oracle={ref['accepted']} phase err t=1 {rel.max():.3e}", flush=True)
+    assert rel.max() < (1e-10 if prec == "f64" else 1e-5), rel.max()
 
 
 def test_sym_n50000_fp64_matches_dense_and_invariants(km):

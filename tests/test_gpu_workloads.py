@@ -3,9 +3,8 @@
 * N=200000 FP32, adjacency given only by its rank-5 factor (the reference's
   -A mode, kuramoto.cu:337-352 / 585-595): the product's factored path
   against the FP32 oracle's factored path on the same inputs (accepted
-  steps within 2% or 1, r(t) within 1e-3, phases at the end no further from
-  the FP64 oracle's run of the same method than twice the FP32 oracle's
-  distance + 1e-5), and the same matrix stored
+  steps within 2% or 1, r(t) within 1e-3, phases at t=1 and t=2 within
+  1e-5 relative to max(1, |theta|)), and the same matrix stored
   explicitly in FP16 (80 GB in HBM) against the factored run (r(t) within
   1e-3: FP16 rounding of the entries perturbs the coupling by ~1e-4
   relative).
@@ -63,10 +62,12 @@ def test_n200000_lowrank_and_f16(km, orc):
     assert abs(g["status"].accepted - ref["accepted"]) <= max(1, int(0.02 * ref["accepted"]))
     assert len(g["r"]) == len(te)
     assert np.max(np.abs(g["r"] - ref["r"])) < 1e-3
-    scale = np.maximum(1.0, np.abs(truth["rows"][-1]))
-    e_gpu = np.max(np.abs(g["rows"][-1].astype(np.float64) - truth["rows"][-1]) / scale)
-    e_orc = np.max(np.abs(ref["rows"][-1].astype(np.float64) - truth["rows"][-1]) / scale)
-    assert e_gpu <= 2 * e_orc + 1e-5, (e_gpu, e_orc)
+    for q in (1, 3):  # t = 1, t = 2
+        r64 = ref["rows"][q].astype(np.float64)
+        e = np.max(np.abs(g["rows"][q].astype(np.float64) - r64) / np.maximum(1.0, np.abs(r64)))
+        print(f"  factored: accepted gpu={g['status'].accepted} oracle={ref['accepted']} "
+              f"phase err t={te[q]} {e:.3e}", flush=True)
+        assert e < 1e-5, (float(te[q]), e)
     f = res["f16"]
     assert len(f["r"]) == len(te)
     assert np.max(np.abs(f["r"] - g["r"])) < 1e-3

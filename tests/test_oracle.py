@@ -149,6 +149,45 @@ def test_rhs_zero_and_symmetric_invariant(orc):
     assert abs(f.sum() - om.sum()) < 1e-12  # symmetric M: sum_i f_i = sum_i w_i
 
 
+def test_fp32_sincos_is_the_rounded_double_function(orc):
+    """FP32 cos / sin of the model are the float32 roundings of the float64
+    functions (numpy's), over the phase range of Cauchy runs (|theta| up to
+    1e5, where one float ulp is 8e-3 rad).  Float-library cosf would differ
+    from these in some last bits."""
+    rng = np.random.default_rng(11)
+    x = np.concatenate([rng.uniform(-4, 4, 100000), rng.uniform(-1e5, 1e5, 100000)]).astype(np.float32)
+    c, s = orc.sincos(x, np.float32)
+    x64 = x.astype(np.float64)
+    assert np.array_equal(c, np.cos(x64).astype(np.float32))
+    assert np.array_equal(s, np.sin(x64).astype(np.float32))
+    c64, s64 = orc.sincos(x64, np.float64)
+    assert np.max(np.abs(c64 - np.cos(x64))) <= 2.3e-16 and np.max(np.abs(s64 - np.sin(x64))) <= 2.3e-16
+
+
+def test_fp32_coupling_sums_round_once(orc):
+    """FP32 coupling sums C_i, S_i accumulate in double and round once to
+    float: they equal the float32 rounding of numpy's float64 sums (another
+    summation order) except where the two double sums straddle a float
+    rounding boundary (none here).  A float accumulation would miss this at
+    the 1e-7 level in most rows."""
+    N = 3000
+    X = orc.gaussian_factor(N, 5, 4).astype(np.float32)
+    w = orc.factor_weights(N, 5, True, np.float32)
+    M = orc.adj_rows(X, w, 0, N)
+    rng = np.random.default_rng(5)
+    th = rng.uniform(-50, 50, N).astype(np.float32)
+    om = rng.standard_normal(N).astype(np.float32)
+    f, cs = orc.rhs(orc.Adj(0, M=M), om, 3.0, th, dtype=np.float32)
+    c, s = orc.sincos(th, np.float32)
+    M64 = M.astype(np.float64)
+    assert np.array_equal(cs[0::2], (M64 @ c.astype(np.float64)).astype(np.float32))
+    assert np.array_equal(cs[1::2], (M64 @ s.astype(np.float64)).astype(np.float32))
+    f32acc = np.zeros(N, np.float32)
+    for j in range(N):  # float accumulation, column order
+        f32acc += M[:, j] * c[j]
+    assert np.count_nonzero(f32acc != cs[0::2]) > N // 2
+
+
 @pytest.mark.parametrize("K", [0, 1, 5, 12])
 def test_rhs_factored_equals_dense(orc, K):
     N = 257
