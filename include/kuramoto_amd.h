@@ -13,7 +13,8 @@
  *     (kmt_advance + kmt_read_*) instead of a step_eval callback;
  *   - every entry point returns a status code (KMT_OK = 0); the message of
  *     the last failure on the calling thread is kmt_last_error().
- * All array arguments are HOST pointers; sizes are element counts.  Real
+ * All array arguments are HOST pointers (kmt_rhs_noalloc's alone are device
+ * pointers); sizes are element counts.  Real
  * arrays use the context precision: float for KMT_F32, double for KMT_F64.
  * Adjacency layout everywhere: row-major, element [i*ld + j] multiplies
  * oscillator j in the equation of oscillator i (README.md:50, kuramoto.cu:113).
@@ -147,6 +148,19 @@ int kmt_get_adjacency_scale(kmt_ctx *ctx, double *scale);
 /* dydt (kuramoto.cu:118-126): f = RHS(theta); optionally the couplings
  * (C_i, S_i) interleaved, 2N values (makecoupling, kuramoto.cu:104-116). */
 int kmt_rhs(kmt_ctx *ctx, const void *theta, void *f, void *couplings);
+/* kmt_rhs without any allocation, on DEVICE vectors: theta_dev (N values)
+ * in, f_dev (N values) and/or couplings_dev (2N values, interleaved C_i,
+ * S_i) out, launched on `stream` (a hipStream_t; NULL: the context's own
+ * stream) with no synchronisation; the caller synchronises before reading
+ * the outputs.  No counterpart in the reference: this is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The (cos, sin) vector lives in
+ * a fixed 4 MiB device array of the library, one per GPU, so N <= 524288
+ * (FP32) / 262144 (FP64); larger contexts get KMT_ERR_ARG.  All calls on one
+ * GPU share that array, and the factored (-A) path also uses the context's
+ * partial-sum buffers: calls that overlap on one GPU (other streams, other
+ * host threads) race.  Results equal kmt_rhs's bit for bit (same kernels). */
+int kmt_rhs_noalloc(kmt_ctx *ctx, const void *theta_dev, void *f_dev, void *couplings_dev, void *stream);
 /* Order parameter r = |mean exp(i theta)| (kuramoto.cu:216-220, :592-597). */
 int kmt_order(kmt_ctx *ctx, const void *theta, double *r);
 

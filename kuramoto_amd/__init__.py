@@ -82,6 +82,7 @@ _SIGNATURES = {
     "kmt_get_adjacency_rows": ([_P, _L, _L, _P], _I),
     "kmt_get_adjacency_scale": ([_P, _P], _I),
     "kmt_rhs": ([_P, _P, _P, _P], _I),
+    "kmt_rhs_noalloc": ([_P, _P, _P, _P, _P], _I),
     "kmt_order": ([_P, _P, _P], _I),
     "kmt_begin": ([_P, _P, _I, _D, _I, _I], _I),
     "kmt_advance": ([_P, _I, _P], _I),
@@ -281,6 +282,11 @@ class Integrator:
         cp = np.zeros(2 * self.N, self.dtype) if couplings else None
         _check(self.lib.kmt_rhs(self.ctx, _ptr(th), _ptr(f), _ptr(cp)), "kmt_rhs")
         return (f, cp) if couplings else f
+
+    def rhs_noalloc(self, theta_dev, f_dev=None, couplings_dev=None, stream=None):
+        """kmt_rhs_noalloc on device pointers (ints): no allocation and no
+        synchronisation; the caller synchronises `stream` before reading."""
+        _check(self.lib.kmt_rhs_noalloc(self.ctx, theta_dev, f_dev, couplings_dev, stream), "kmt_rhs_noalloc")
 
     def couplings(self, theta):
         th = self._vec(theta, self.N)

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -81,6 +82,44 @@ struct kmt_ctx {
   int precision;
   EngineBase *eng;
 };
+
+// ---------------------------------------------------------------------
+// Scratch of kmt_rhs_noalloc: the (cos, sin) vector of one RHS, 2 ldv
+// values of the working precision (4 MiB: N <= 524288 in FP32, 262144 in
+// FP64).  A file-scope device array, so the entry point allocates nothing;
+// every GPU holds its own copy until the process ends.  In host code the
+// array's name is only a host-side stand-in: its device address on each
+// GPU comes from hipGetSymbolAddress, once per device with that device
+// current, and only that address is passed to kernels.  Every call on a
+// GPU shares that GPU's copy (calls that overlap on one GPU race); each
+// call rewrites all of the values it reads, zeros past N included.
+// ---------------------------------------------------------------------
+static constexpr size_t kRhsScratchBytes = size_t(4) << 20;
+static __device__ __align__(256) unsigned char scratch[kRhsScratchBytes];
+
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kMaxDevices] = {};
+
+static int rhs_scratch_ptr(int device, void **out) {
+  if (device < 0 || device >= kMaxDevices) return kmt_fail(KMT_ERR_ARG, "device %d out of range", device);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    int cur = 0;
+    HIP_TRY(hipGetDevice(&cur));
+    HIP_TRY(hipSetDevice(device));
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    const hipError_t e2 = hipSetDevice(cur);
+    if (e != hipSuccess || !ptr)
+      return kmt_fail(KMT_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d failed: %s", device,
+                      hipGetErrorString(e));
+    if (e2 != hipSuccess) return kmt_fail(KMT_ERR_HIP, "hipSetDevice(%d) failed: %s", cur, hipGetErrorString(e2));
+    g_scratch_ptr[device] = ptr;
+  }
+  *out = g_scratch_ptr[device];
+  return KMT_OK;
+}
 
 // ---------------------------------------------------------------------
 // host generators
@@ -269,6 +308,19 @@ int kmt_rhs(kmt_ctx *ctx, const void *theta, void *f, void *couplings) {
   CTX_CHECK;
   if (!theta || (!f && !couplings)) return fail(KMT_ERR_ARG, "theta is NULL or no output requested");
   return ctx->eng->rhs(theta, f, couplings);
+}
+int kmt_rhs_noalloc(kmt_ctx *ctx, const void *theta_dev, void *f_dev, void *couplings_dev, void *stream) {
+  CTX_CHECK;
+  if (!theta_dev || (!f_dev && !couplings_dev)) return fail(KMT_ERR_ARG, "theta_dev is NULL or no output requested");
+  const size_t need = ctx->eng->rhs_scratch_bytes();
+  if (need > kRhsScratchBytes)
+    return fail(KMT_ERR_ARG,
+                "kmt_rhs_noalloc: this context's (cos, sin) vector needs %zu bytes, more than the library's "
+                "%zu-byte scratch (N <= %zu in FP32, %zu in FP64)",
+                need, kRhsScratchBytes, kRhsScratchBytes / 8, kRhsScratchBytes / 16);
+  void *cs = nullptr;
+  KMT_TRY(rhs_scratch_ptr(ctx->eng->device_index(), &cs));
+  return ctx->eng->rhs_device(theta_dev, f_dev, couplings_dev, cs, (hipStream_t)stream);
 }
 int kmt_order(kmt_ctx *ctx, const void *theta, double *r) {
   CTX_CHECK;

@@ -39,6 +39,9 @@ struct EngineBase {
   virtual int use_factored(int) = 0;
   virtual int get_adjacency_rows(long, long, void *) = 0;
   virtual int rhs(const void *, void *, void *) = 0;
+  virtual int rhs_device(const void *, void *, void *, void *, hipStream_t) = 0;
+  virtual size_t rhs_scratch_bytes() const = 0;
+  virtual int device_index() const = 0;
   virtual int order(const void *, double *) = 0;
   virtual int begin(const double *, int, double, int, int) = 0;
   virtual int advance(int, kmt_status *) = 0;
@@ -885,6 +888,27 @@ template <class R> struct Engine : EngineBase {
     HIP_TRY(hipStreamSynchronize(stream));
     return KMT_OK;
   }
+
+  // kmt_rhs on device vectors: theta, f and the couplings stay where the
+  // caller keeps them, the (cos, sin) vector goes to the caller's buffer cs
+  // (rhs_scratch_bytes(), zero past N), and the launches go to stream s
+  // (nullptr: the context's stream) without a synchronisation.  Nothing is
+  // allocated; the factored path uses the context's partial-sum buffers.
+  int rhs_device(const void *th, void *f, void *cpl, void *cs, hipStream_t s) override {
+    if (!coupling_ready()) return fail(KMT_ERR_STATE, "no adjacency: call kmt_set_adjacency, kmt_build_adjacency or kmt_set_factor + kmt_use_factored");
+    const hipStream_t own = stream;
+    stream = s ? s : own;
+    sincos_pad_kernel<R><<<(unsigned)((ldv + kBlock - 1) / kBlock), kBlock, 0, stream>>>((const R *)th, N, ldv, (R *)cs);
+    Vecs<R> v = vecs((const R *)cs, nullptr, 0);
+    v.f_out = (R *)f;
+    v.cs_pairs = (R *)cpl;
+    launch_by_kind(f ? KIND_RHS : KIND_COUPLING, v);
+    stream = own;
+    HIP_TRY(hipGetLastError());
+    return KMT_OK;
+  }
+  size_t rhs_scratch_bytes() const override { return sizeof(R) * 2 * (size_t)ldv; }
+  int device_index() const override { return device; }
 
   int order(const void *th, double *r) override {
     HIP_TRY(hipMemcpyAsync(vec_scr, th, sizeof(R) * N, hipMemcpyHostToDevice, stream));

@@ -924,6 +924,18 @@ __global__ __launch_bounds__(kBlock) void sincos_kernel(const R *y, long N, long
   }
 }
 
+// The same over the padded length: zeros at [N, ldv) of both halves, for a
+// (cos, sin) buffer that other calls with other N share.
+template <class R>
+__global__ __launch_bounds__(kBlock) void sincos_pad_kernel(const R *y, long N, long ldv, R *cs) {
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (i < ldv) {
+    const bool in = i < N;
+    cs[i] = in ? rcos<R>(y[i]) : (R)0;
+    cs[ldv + i] = in ? rsin<R>(y[i]) : (R)0;
+  }
+}
+
 template <class R>
 __global__ __launch_bounds__(kBlock) void order_partial_kernel(const R *y, long N, double *part) {
   __shared__ double sh[kBlock];
