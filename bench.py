@@ -131,10 +131,10 @@ def cpu_baseline(cfg, max_steps):
                        f"OpenMP threads={orc.num_threads()}")
 
 
-def batch_to_time(name, proposer, t_end):
+def batch_to_time(name, proposer, t_end, tile=None):
     """Whole integration of every instance from t=0 to t_end (host wall
     clock around the advance loop, adjacency already resident)."""
-    bt, Ns, _ = make_batch(name, proposer)
+    bt, Ns, _ = make_batch(name, proposer, tile)
     dt = np.float32(BATCH_CONFIGS[name]["dt"])
     te = (dt * np.arange(1, int(np.float32(t_end) / dt) + 1)).astype(np.float64)
     t0 = time.perf_counter()
@@ -177,9 +177,9 @@ def sequential_plain_to_time(name, t_end):
     return dict(seconds=secs, r=rs, **tot)
 
 
-def batch_run(name, steps, warmup, proposer):
+def batch_run(name, steps, warmup, proposer, tile=None):
     cfg = BATCH_CONFIGS[name]
-    bt, Ns, tune = make_batch(name, proposer)
+    bt, Ns, tune = make_batch(name, proposer, tile)
     dt = np.float32(cfg["dt"])
     te = (dt * np.arange(1, int(np.float32(cfg["t1"]) / dt) + 1)).astype(np.float64)
     bt.begin(te, cfg["t1"], ring_rows=1024)
@@ -213,7 +213,8 @@ def batch_main(args):
     import kuramoto_amd as km
 
     cfg = BATCH_CONFIGS[args.config]
-    r = batch_run(args.config, args.steps, args.warmup, proposer=not cfg["shared"])
+    tile = tuple(int(v) for v in args.pin_tile.split(",")) if args.pin_tile else None
+    r = batch_run(args.config, args.steps, args.warmup, proposer=not cfg["shared"], tile=tile)
     st, st0 = r["st"], r["st0"]
     secs = r["region_ms"] / 1e3
     osc_steps = st.osc_steps - st0.osc_steps
@@ -233,37 +234,40 @@ def batch_main(args):
                  "tile": {"workgroup_variant": r["tune"][0], "split_k": r["tune"][1]}}
         # time to solution: every instance integrated from t=0 to t_end by the
         # batch (standard controller and learned proposer) and by 64 plain
-        # runs one after another
+        # runs one after another (t_end = 0: skipped)
         t_end = args.batch_t_end
-        runs = {m: batch_to_time(args.config, m == "batched_learned_proposer", t_end)
-                for m in ("batched_standard_controller", "batched_learned_proposer")}
-        runs["sequential_plain_fp32"] = sequential_plain_to_time(args.config, t_end)
-        plain_r = runs["sequential_plain_fp32"]["r"]
-        dr = {m: max(float(np.max(np.abs(a - b))) for a, b in zip(runs[m]["r"], plain_r))
-              for m in ("batched_standard_controller", "batched_learned_proposer")}
-        for m in runs:
-            runs[m].pop("r")
-        seq_s = runs["sequential_plain_fp32"]["seconds"]
-        extra.update({f"whole_run_to_t{t_end:g}": runs,
-                      "max_abs_dr_vs_sequential_plain": dr,
-                      "time_to_solution_speedup_vs_sequential_plain": {
-                          m: seq_s / runs[m]["seconds"]
-                          for m in ("batched_standard_controller", "batched_learned_proposer")}})
+        if t_end > 0:
+            runs = {m: batch_to_time(args.config, m == "batched_learned_proposer", t_end, tile)
+                    for m in ("batched_standard_controller", "batched_learned_proposer")}
+            runs["sequential_plain_fp32"] = sequential_plain_to_time(args.config, t_end)
+            plain_r = runs["sequential_plain_fp32"]["r"]
+            dr = {m: max(float(np.max(np.abs(a - b))) for a, b in zip(runs[m]["r"], plain_r))
+                  for m in ("batched_standard_controller", "batched_learned_proposer")}
+            for m in runs:
+                runs[m].pop("r")
+            seq_s = runs["sequential_plain_fp32"]["seconds"]
+            extra.update({f"whole_run_to_t{t_end:g}": runs,
+                          "max_abs_dr_vs_sequential_plain": dr,
+                          "time_to_solution_speedup_vs_sequential_plain": {
+                              m: seq_s / runs[m]["seconds"]
+                              for m in ("batched_standard_controller", "batched_learned_proposer")}})
     else:
         alg = sum(2 * n * n + 3 * 4 * n for n in Ns)
         # the learned proposer changes how many attempts reach a given t, so
         # the two controllers are compared on whole runs to the same end time
         t_end = args.batch_t_end
-        runs = {m: batch_to_time(args.config, m == "learned_proposer", t_end)
-                for m in ("standard_controller", "learned_proposer")}
-        dr = max(float(np.max(np.abs(a - b))) for a, b in zip(runs["standard_controller"]["r"],
-                                                               runs["learned_proposer"]["r"]))
-        for m in runs:
-            runs[m].pop("r")
-            runs[m]["osc_steps_per_s"] = runs[m]["osc_steps"] / runs[m]["seconds"]
-        extra = {f"whole_run_to_t{t_end:g}": runs, "max_abs_dr_learned_vs_standard": dr,
-                 "speedup_learned_vs_standard": runs["standard_controller"]["seconds"]
-                 / runs["learned_proposer"]["seconds"], "sum_N": int(sum(Ns))}
+        extra = {"sum_N": int(sum(Ns))}
+        if t_end > 0:
+            runs = {m: batch_to_time(args.config, m == "learned_proposer", t_end)
+                    for m in ("standard_controller", "learned_proposer")}
+            dr = max(float(np.max(np.abs(a - b))) for a, b in zip(runs["standard_controller"]["r"],
+                                                                   runs["learned_proposer"]["r"]))
+            for m in runs:
+                runs[m].pop("r")
+                runs[m]["osc_steps_per_s"] = runs[m]["osc_steps"] / runs[m]["seconds"]
+            extra.update({f"whole_run_to_t{t_end:g}": runs, "max_abs_dr_learned_vs_standard": dr,
+                          "speedup_learned_vs_standard": runs["standard_controller"]["seconds"]
+                          / runs["learned_proposer"]["seconds"]})
     achieved = alg / (r["coupling_ms"] / 1e3)
     copy_gbps, read_gbps = km.bandwidth_probe(8 << 30, 5)
     kname = "bgemm_kernel" if cfg["shared"] else "bgemv_kernel"
@@ -296,7 +300,9 @@ def main():
                     help="accepted steps of the CPU baseline sample: a fixed window from t=0 (0: skip)")
     ap.add_argument("--batch-t-end", type=float, default=10.0,
                     help="batched workloads: end time of the whole runs that compare the step controllers "
-                         "(and, for the shared batch, 64 sequential plain runs)")
+                         "(and, for the shared batch, 64 sequential plain runs); 0 skips them")
+    ap.add_argument("--pin-tile", default=None,
+                    help="shared batch: 'variant,split_k' of the MFMA product instead of the autotuner's choice")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (the row-sharded "

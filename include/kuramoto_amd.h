@@ -213,7 +213,9 @@ int kmt_bandwidth_probe(int device, size_t bytes, int reps, double *copy_gbps, d
  * follow the variant's workgroups, so its last bits (and h's) can differ
  * between variants.  Without this call kmt_begin uses a fixed choice (4 rows,
  * plain loads when N >= 4096 and the matrix is <= 128 MB; else 8 rows,
- * non-temporal), so that every context and process gives the same bits.  KMT_ADJ_SYM has one
+ * non-temporal), so that every context and process gives the same bits.
+ * Calling it (mid-run, or in only one of two contexts) therefore breaks
+ * bit-exact restart against contexts that use the fixed choice.  KMT_ADJ_SYM has one
  * form and is not tuned.  No reference counterpart (cuBLAS chooses its own
  * gemv kernel). */
 int kmt_autotune(kmt_ctx *ctx, int reps, double *variant_ms);

@@ -853,7 +853,10 @@ struct kmt_batch {
   int log_cap = 0;
 
   ~kmt_batch() { release(); }
+  // Destroy the captured attempt graphs (after the stream has drained, as
+  // every caller already ensures).
   void drop_graphs() {
+    if ((g1 || g8) && stream) (void)hipStreamSynchronize(stream);
     if (g1) (void)hipGraphExecDestroy(g1);
     if (g8) (void)hipGraphExecDestroy(g8);
     g1 = g8 = nullptr;
@@ -1514,6 +1517,7 @@ struct kmt_batch {
   }
 
   int set_log(int cap) {
+    HIP_TRY(hipStreamSynchronize(stream));  // the log below may be in use by queued work
     drop_graphs();
     if (log) { (void)hipFree(log); log = nullptr; }
     if (log_count) { (void)hipFree(log_count); log_count = nullptr; }

@@ -127,7 +127,11 @@ template <class R> struct Engine : EngineBase {
     stream = nullptr;
   }
 
+  // Destroy the captured attempt graphs.  Every entry point that launches
+  // them synchronises the stream before it returns, so none is in flight
+  // here; the synchronisation keeps that true for any later caller.
   void drop_graphs() {
+    if ((g1 || g8) && stream) (void)hipStreamSynchronize(stream);
     if (g1) (void)hipGraphExecDestroy(g1);
     if (g8) (void)hipGraphExecDestroy(g8);
     g1 = g8 = nullptr;
@@ -448,6 +452,7 @@ template <class R> struct Engine : EngineBase {
 
   int set_factor(const void *Xv, const void *wv, int k_) override {
     if (k_ < 0) return fail(KMT_ERR_ARG, "rank K must be >= 0");
+    HIP_TRY(hipStreamSynchronize(stream));  // the buffers below may be in use by queued work
     drop_graphs();
     if (X) { (void)hipFree(X); X = nullptr; }
     if (w) { (void)hipFree(w); w = nullptr; }
@@ -774,8 +779,11 @@ template <class R> struct Engine : EngineBase {
   // non-temporal adjacency loads {off, on} x grid {one workgroup per row
   // group, balanced: one resident round with contiguous equal row ranges},
   // each timed on this context's own matrix (stage 3 on scratch buffers).
-  // Every variant sums each row in the same order, so the choice changes
-  // speed only, never results.
+  // Every variant sums each row in the same order (the RHS is bit-identical),
+  // but the stage-7 error partials follow the variant's workgroups, so the
+  // last bits of the error norm and of h can differ from the default rule's:
+  // a context tuned here does not restart bit-exactly against one that was
+  // not (or that picked another variant).
   int autotune(int reps) {
     if (factored || !A || sym()) return KMT_OK;  // one form of the symmetric kernels
     if (!k_scr) {

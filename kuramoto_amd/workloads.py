@@ -68,7 +68,9 @@ def uneven_sizes(B, seed=147):
     return [int(5000 * rng.integers(1, 9)) for _ in range(B)]
 
 
-def make_batch(name, proposer):
+def make_batch(name, proposer, tile=None):
+    """The batch of workload `name`; the shared product's tile is autotuned,
+    or pinned to tile = (workgroup variant, split-K)."""
     import kuramoto_amd as km
 
     cfg = BATCH[name]
@@ -84,7 +86,11 @@ def make_batch(name, proposer):
         if not cfg["shared"]:
             X = km.factor(Ns[b], 5, cfg["seeds"][b], gaussian=True).astype(np.float32)
             bt.set_factor(b, X, km.factor_weights(km.F32, Ns[b], 5, gaussian=True))
-    tune = bt.autotune(3) if cfg["shared"] else None
+    if cfg["shared"] and tile is not None:
+        bt.select_tile(*tile)
+        tune = (tile[0], tile[1], None)
+    else:
+        tune = bt.autotune(3) if cfg["shared"] else None
     if proposer:
         w = np.load(os.path.join(HERE, "proposer_weights.npz"))["w16"]
         bt.set_proposer(w, True)
