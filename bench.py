@@ -67,15 +67,18 @@ def gpu_run(cfg, steps, warmup):
     dtype = np.float32 if p == km.F32 else np.float64
     N, K = cfg["N"], cfg["K"]
     theta0, omega = km.draws(p, N, cfg["seed"], normal=cfg["normal"], freq=1.0, init=cfg["init"])
-    X = km.factor(N, K, cfg["seed"], gaussian=True).astype(dtype)
-    w = km.factor_weights(p, N, K, gaussian=True)
     dt = dtype(cfg["dt"])
     te = (dt * np.arange(1, int(dtype(cfg["t1"]) / dt) + 1)).astype(np.float64)
     it = km.Integrator(N, p, atol=cfg["atol"], c1=cfg["c1"])
     it.set_omega(omega)
-    it.set_factor(X, w)
     t0 = time.time()
-    if cfg["storage"] == "factored":
+    if cfg["storage"] == "signs":
+        it.set_factor_sign(K, cfg["seed"])
+    else:
+        X = km.factor(N, K, cfg["seed"], gaussian=True).astype(dtype)
+        it.set_factor(X, km.factor_weights(p, N, K, gaussian=True))
+        t0 = time.time()
+    if cfg["storage"] in ("factored", "signs"):
         it.use_factored("contract")
     elif cfg["storage"] == "rebuild":
         it.use_factored("rebuild")
@@ -111,11 +114,15 @@ def cpu_baseline(cfg, max_steps):
     p = km.F32 if cfg["prec"] == "f32" else km.F64
     dtype = np.float32 if p == km.F32 else np.float64
     theta0, omega = orc.draws(N, cfg["seed"], cfg["normal"], 1.0, cfg["init"], dtype=dtype)
-    X = orc.gaussian_factor(N, K, cfg["seed"]).astype(dtype)
-    w = orc.factor_weights(N, K, True, dtype=dtype)
+    signs = cfg["storage"] == "signs"
+    if signs and N * K > 2e8:
+        return dict(value=None, unit="osc*steps/s", cores=orc.num_threads(), kind="port",
+                    sample=f"not run: the oracle's -A path holds X as an N x K array ({4 * N * K / 1e9:.0f} GB in FP32)")
+    X = (orc.rademacher_factor(N, K, cfg["seed"]) if signs else orc.gaussian_factor(N, K, cfg["seed"])).astype(dtype)
+    w = orc.factor_weights(N, K, not signs, dtype=dtype)
     # N^2 does not fit the host for the N=200000 cases: the oracle runs the
     # reference's own -A path (coupling through the factor) there
-    lowrank = cfg["storage"] in ("factored", "f16", "rebuild")
+    lowrank = cfg["storage"] in ("factored", "f16", "rebuild", "signs")
     adj = orc.Adj(2, X=X, w=w) if lowrank else orc.Adj(0, M=orc.adj_rows(X, w, 0, N))
     dt = dtype(cfg["dt"])
     te = (dt * np.arange(1, int(dtype(cfg["t1"]) / dt) + 1)).astype(dtype)
@@ -290,6 +297,58 @@ def batch_main(args):
     print(json.dumps(out), flush=True)
 
 
+# The reference's own published runs of its programs (volcano.ipynb): the
+# command line, the notebook cell and the "runtime:" and final step count
+# it printed (an unnamed CUDA GPU with 25 GB; whole-process time: adjacency
+# generation, integration, every output file).
+PUBLISHED = [
+    dict(prog="kuramoto", args="-N 10000 -K 10000 -c 1.75 -t 100 -d 0.01 -s 1 -nv", runtime=26.206418, steps=846,
+         source="volcano.ipynb:47-52 (dense FP32)"),
+    dict(prog="kuramoto", args="-N 10000 -K 10000 -c 1.75 -t 100 -d 0.01 -s 1 -nvA", runtime=93.887063, steps=846,
+         source="volcano.ipynb:55-60 (-A FP32)"),
+    dict(prog="kuramoto_64", args="-N 10000 -K 10000 -c 1.75 -t 100 -d 0.01 -s 1 -nv", runtime=60.521239,
+         steps=845, source="volcano.ipynb:63-68 (dense FP64)"),
+    dict(prog="kuramoto", args="-N 100000 -K 100000 -c 3 -t 20 -d 0.02 -s 1 -D1 -nvA", runtime=422.154478,
+         steps=230, source="volcano.ipynb:319-346 (-A FP32, N = K = 10^5)"),
+    dict(prog="kuramoto", args="-N 100000 -K 6 -c 3 -t 20 -d 0.02 -s 1 -D1 -nvA", runtime=13.55, steps=120,
+         source="volcano.ipynb:547-573 (-A FP32, K = 6)"),
+]
+
+
+def published_main(args):
+    """Each published command through this build's drop-in programs, in a
+    scratch directory (files deleted after each run): the programs' own
+    "runtime:" line and accepted steps against the reference's."""
+    import shutil
+    import subprocess
+    import tempfile
+
+    rows = []
+    for p in PUBLISHED:
+        work = tempfile.mkdtemp(prefix="kmt_pub_", dir=args.scratch)
+        try:
+            cmd = [os.path.join(ROOT, "kuramoto_amd", "bin", p["prog"])] + p["args"].split() + [
+                os.path.join(work, "run")]
+            t0 = time.perf_counter()
+            out = subprocess.run(cmd, capture_output=True, text=True, timeout=900)
+            wall = time.perf_counter() - t0
+            text = out.stdout + open(os.path.join(work, "run.out")).read()
+            rt = [float(l.split()[1]) for l in text.splitlines() if l.startswith("runtime:")]
+            st = [int(l.split()[1]) for l in text.splitlines() if l.startswith("steps:")]
+            sizes = sum(os.path.getsize(os.path.join(work, f)) for f in os.listdir(work))
+        finally:
+            shutil.rmtree(work, ignore_errors=True)
+        rows.append({"command": f"{p['prog']} {p['args']}", "source": p["source"], "rc": out.returncode,
+                     "runtime_s": rt[-1] if rt else None, "process_wall_s": wall,
+                     "accepted_steps": st[-1] if st else None, "reference_runtime_s": p["runtime"],
+                     "reference_steps": p["steps"], "output_bytes": sizes,
+                     "speedup_vs_reference": (p["runtime"] / rt[-1]) if rt else None})
+        print(f"published: {rows[-1]}", file=sys.stderr, flush=True)
+    print(json.dumps({"metric": "whole-process runtime of the reference's published commands (s)",
+                      "unit": "s", "higher_is_better": False, "n_gpus": 1, "data": "synthetic, as the programs draw",
+                      "runs": rows}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -303,7 +362,13 @@ def main():
                          "(and, for the shared batch, 64 sequential plain runs); 0 skips them")
     ap.add_argument("--pin-tile", default=None,
                     help="shared batch: 'variant,split_k' of the MFMA product instead of the autotuner's choice")
+    ap.add_argument("--published", action="store_true",
+                    help="run the reference's published command lines through the drop-in programs instead")
+    ap.add_argument("--scratch", default=None, help="--published: directory for the programs' output files")
     args = ap.parse_args()
+    if args.published:
+        published_main(args)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (the row-sharded "
               "multi-GPU layout is a design only)", file=sys.stderr)
@@ -320,6 +385,9 @@ def main():
     if cfg["storage"] == "factored":
         # per RHS: X read twice (N K s_y each) + theta-derived vectors in, k out, omega
         alg_bytes = 2 * N * cfg["K"] * s_y + N * (2 * s_y + s_y)
+    elif cfg["storage"] == "signs":
+        # per RHS: the packed signs read twice (N K / 8 bytes each pass)
+        alg_bytes = 2 * N * cfg["K"] // 8 + N * (2 * s_y + s_y)
     elif cfg["storage"].startswith("sym"):
         # a symmetric matrix carries N (N + 1) / 2 distinct entries
         alg_bytes = r["elem_bytes"] * N * (N + 1) // 2 + N * (2 * s_y + s_y)
@@ -328,6 +396,16 @@ def main():
     achieved = alg_bytes / (r["stage_ms"] / 1e3)
     roof = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
             "frac": achieved / HBM_PEAK}
+    if cfg["storage"] == "signs":
+        # two passes of signed sums over the N x K signs per RHS, each entry
+        # one signed addition per vector (cos, sin): 4 N K FP64 additions
+        # (the reference's m1 / m2 loops), issued as 4 VALU instructions per
+        # entry and pass (bit extract, conversion, two fmas)
+        flops = 4 * N * cfg["K"]
+        tf = flops / (r["stage_ms"] / 1e3) / 1e12
+        roof = {"bound": "valu", "achieved": tf, "peak": 78.6, "unit": "TFLOP/s", "frac": tf / 78.6,
+                "flops_per_launch": flops, "valu_instructions_per_launch": 8 * N * cfg["K"],
+                "valu_issue_frac": 8 * N * cfg["K"] / (r["stage_ms"] / 1e3) / (256 * 64 * 2.4e9)}
     if cfg["storage"] == "rebuild":
         # (K products + K fmas to form M_ij, 2 fmas into C and S) per pair:
         # 3K + 4 flops in 2K + 3 VALU instructions per (i, j) (M_ij converted
@@ -354,6 +432,7 @@ def main():
         "config": {"workload": f"{args.config}: N={N}, K={cfg['K']}, "
                                + ("low-rank factor only (-A)" if cfg["storage"] == "factored" else
                                   "-A, tiles rebuilt in registers" if cfg["storage"] == "rebuild" else
+                                  "-A, volcano signs packed 1 bit per entry" if cfg["storage"] == "signs" else
                                   f"symmetric adjacency, upper block triangle ({r['elem_bytes']} B/elem)"
                                   if cfg["storage"].startswith("sym") else
                                   f"dense adjacency {cfg['storage']} ({r['elem_bytes']} B/elem)")
@@ -366,7 +445,9 @@ def main():
                      "traffic": profiled_traffic(args.config, "lowrank_stage_kernel" if cfg["storage"] == "factored"
                                                  else "sym_tile_kernel" if cfg["storage"].startswith("sym")
                                                  else "stage_kernel"),
-                     "kernel": ("lowrank_stage_kernel (+ gram at stage 2; one RHS)" if cfg["storage"] == "factored"
+                     "kernel": ("sign_sum_kernel x 2 + sign_gram_kernel + sign_stage_kernel (one RHS)"
+                                if cfg["storage"] == "signs" else
+                                "lowrank_stage_kernel (+ gram at stage 2; one RHS)" if cfg["storage"] == "factored"
                                 else "rebuild_stage_kernel (tiles rebuilt in registers)" if cfg["storage"] == "rebuild"
                                 else "sym_tile_kernel + sym_reduce_kernel (one RHS from the upper block triangle)"
                                 if cfg["storage"].startswith("sym")

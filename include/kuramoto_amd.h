@@ -125,6 +125,18 @@ int kmt_set_adjacency(kmt_ctx *ctx, const void *M, int src_precision, long ld, i
 /* Low-rank factor for the adjacency: X (N x K, row-major, context
  * precision) and weights w (K).  Used by kmt_build_adjacency and -A. */
 int kmt_set_factor(kmt_ctx *ctx, const void *X, const void *w, int K);
+/* The reference's -A volcano factor for any rank K, drawn on the device:
+ * X_nm = +-1 from the library's Philox stream (the same signs as
+ * kmt_factor(N, K, seed, 0)) with w_m = (-1)^m / N (makeu1 / makeu2 / m1 /
+ * m2, kuramoto.cu:62-143, which regenerate the signs from Philox counters
+ * and store none), kept packed at one bit per entry plus its transpose
+ * (N K / 4 bytes in all, 2.5 GB at N = K = 10^5) instead of an N x K real
+ * array from the host.  Afterwards kmt_use_factored(ctx, KMT_LOWRANK_CONTRACT)
+ * couples through it (two passes of signed sums in double per RHS, any K),
+ * and kmt_build_adjacency builds the dense matrix from it (native, BF16 or
+ * FP16 storage) with the same fma chain as from a real factor.  Replaces a
+ * previous kmt_set_factor; no register-rebuild form. */
+int kmt_set_factor_sign(kmt_ctx *ctx, int K, uint32_t seed);
 /* Build the dense adjacency on the device from the factor (getadj,
  * kuramoto.cu:145-161): M_ij = sum_m fma(w_m, X_im X_jm, .) in the working
  * precision, stored as `storage`.  Exactly symmetric. */

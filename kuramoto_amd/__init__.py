@@ -77,6 +77,7 @@ _SIGNATURES = {
     "kmt_get_state": ([_P, _P, _P, _P], _I),
     "kmt_set_adjacency": ([_P, _P, _I, _L, _I], _I),
     "kmt_set_factor": ([_P, _P, _P, _I], _I),
+    "kmt_set_factor_sign": ([_P, _I, ctypes.c_uint32], _I),
     "kmt_build_adjacency": ([_P, _I], _I),
     "kmt_use_factored": ([_P, _I], _I),
     "kmt_get_adjacency_rows": ([_P, _L, _L, _P], _I),
@@ -253,6 +254,11 @@ class Integrator:
             raise ValueError("factor must be N x K")
         _check(self.lib.kmt_set_factor(self.ctx, _ptr(X) if K else None, _ptr(w) if K else None, K),
                "kmt_set_factor")
+
+    def set_factor_sign(self, K, seed):
+        """The volcano +-1 factor of rank K drawn on the device (packed bits),
+        w_m = (-1)^m / N."""
+        _check(self.lib.kmt_set_factor_sign(self.ctx, int(K), int(seed) & 0xFFFFFFFF), "kmt_set_factor_sign")
 
     def build_adjacency(self, storage="native"):
         _check(self.lib.kmt_build_adjacency(self.ctx, STORAGE[storage]), "kmt_build_adjacency")

@@ -286,6 +286,10 @@ int kmt_set_factor(kmt_ctx *ctx, const void *X, const void *w, int K) {
   if (K > 0 && (!X || !w)) return fail(KMT_ERR_ARG, "X or w is NULL");
   return ctx->eng->set_factor(X, w, K);
 }
+int kmt_set_factor_sign(kmt_ctx *ctx, int K, uint32_t seed) {
+  CTX_CHECK;
+  return ctx->eng->set_factor_sign(K, seed);
+}
 int kmt_build_adjacency(kmt_ctx *ctx, int storage) {
   CTX_CHECK;
   return ctx->eng->build_adjacency(storage);

@@ -16,6 +16,7 @@
 #include "../../include/kuramoto_amd.h"
 #include "kmt_common.hpp"
 #include "kmt_kernels.hpp"
+#include "kmt_sign.hpp"
 #include "kmt_sym.hpp"
 
 namespace kmt {
@@ -35,6 +36,7 @@ struct EngineBase {
   virtual int get_state(void *, double *, double *) = 0;
   virtual int set_adjacency(const void *, int, long, int) = 0;
   virtual int set_factor(const void *, const void *, int) = 0;
+  virtual int set_factor_sign(int, uint32_t) = 0;
   virtual int build_adjacency(int) = 0;
   virtual int use_factored(int) = 0;
   virtual int get_adjacency_rows(long, long, void *) = 0;
@@ -89,6 +91,13 @@ template <class R> struct Engine : EngineBase {
   SymPair<double> *sym_prow = nullptr, *sym_pcol = nullptr;
   double *lr_part = nullptr, *lr_G = nullptr;
   int lr_nblk = 0, lr_grid = 0;
+  // packed +-1 factor (kmt_set_factor_sign): sign bits [N][s_ldw] and their
+  // transpose [32 s_kw][s_ldwT]; pass-1 / pass-2 partial sums and G
+  int sign = 0;
+  uint32_t *sbits = nullptr, *sbitsT = nullptr;
+  long s_kw = 0, s_ldw = 0, s_nw = 0, s_ldwT = 0, s_per1 = 1, s_per2 = 1;
+  int s_split1 = 1, s_split2 = 1;
+  double *s_part1 = nullptr, *s_part2 = nullptr, *s_G = nullptr;
   std::vector<R> X_host, w_host;
 
   // run
@@ -113,7 +122,9 @@ template <class R> struct Engine : EngineBase {
     R *vecs[] = {y, ylast, ynew, omega, kbuf, cs[0], cs[1], cs_scr, vec_scr, f_scr, pairs, k_scr, X, w, XT, t_eval, ring, r_ring, times};
     for (R *p : vecs)
       if (p) (void)hipFree(p);
-    double *dv[] = {err_part, ord_scr, lr_part, lr_G, ord_part};
+    double *dv[] = {err_part, ord_scr, lr_part, lr_G, ord_part, s_part1, s_part2, s_G};
+    if (sbits) (void)hipFree(sbits);
+    if (sbitsT) (void)hipFree(sbitsT);
     for (double *p : dv)
       if (p) (void)hipFree(p);
     if (st) (void)hipFree(st);
@@ -459,6 +470,7 @@ template <class R> struct Engine : EngineBase {
     if (XT) { (void)hipFree(XT); XT = nullptr; }
     if (lr_part) { (void)hipFree(lr_part); lr_part = nullptr; }
     if (lr_G) { (void)hipFree(lr_G); lr_G = nullptr; }
+    drop_sign();
     K = k_;
     const long kk = std::max(K, 1);
     X_host.assign((const R *)Xv, (const R *)Xv + (size_t)N * K);
@@ -484,6 +496,90 @@ template <class R> struct Engine : EngineBase {
     return KMT_OK;
   }
 
+  void drop_sign() {
+    uint32_t *b[] = {sbits, sbitsT};
+    for (uint32_t *p : b)
+      if (p) (void)hipFree(p);
+    double *d[] = {s_part1, s_part2, s_G};
+    for (double *p : d)
+      if (p) (void)hipFree(p);
+    sbits = sbitsT = nullptr;
+    s_part1 = s_part2 = s_G = nullptr;
+    sign = 0;
+  }
+
+  // splits of the outer rows so that a pass runs about 2048 waves (8 per
+  // CU): words / 256 workgroups of 4 waves per split, at least 32 rows each
+  static int sign_splits(long nwords, long n_outer, long *per) {
+    const long wg = (nwords + kBlock - 1) / kBlock;
+    long sp = std::max<long>(1, 512 / std::max<long>(wg, 1));
+    sp = std::min<long>(sp, std::max<long>(1, n_outer / 32));
+    *per = std::max<long>(1, (n_outer + sp - 1) / sp);
+    return (int)((n_outer + *per - 1) / *per);
+  }
+
+  // The volcano factor X_nm = +-1 of the library's sign stream, packed, and
+  // w_m = (-1)^m / N (makeu1 / makeu2 / m1 / m2, kuramoto.cu:62-143).
+  int set_factor_sign(int k_, uint32_t seed) override {
+    if (k_ < 0) return fail(KMT_ERR_ARG, "rank K must be >= 0");
+    HIP_TRY(hipStreamSynchronize(stream));
+    drop_graphs();
+    if (X) { (void)hipFree(X); X = nullptr; }
+    if (w) { (void)hipFree(w); w = nullptr; }
+    if (XT) { (void)hipFree(XT); XT = nullptr; }
+    if (lr_part) { (void)hipFree(lr_part); lr_part = nullptr; }
+    if (lr_G) { (void)hipFree(lr_G); lr_G = nullptr; }
+    drop_sign();
+    if (factored == KMT_LOWRANK_REBUILD) factored = KMT_LOWRANK_CONTRACT;
+    K = k_;
+    X_host.clear();
+    w_host.assign((size_t)K, (R)0);
+    for (int m = 0; m < K; ++m) w_host[m] = (R)(((m & 1) ? -1.0 : 1.0) / (double)N);
+    HIP_TRY(hipMalloc((void **)&w, sizeof(R) * std::max(K, 1)));
+    if (K > 0) HIP_TRY(hipMemcpyAsync(w, w_host.data(), sizeof(R) * K, hipMemcpyHostToDevice, stream));
+    s_kw = (K + 31) / 32;
+    s_ldw = round_up(std::max<long>(s_kw, 1), 4);
+    s_nw = (N + 31) / 32;
+    s_ldwT = round_up(s_nw, 64);
+    hipError_t e = hipMalloc((void **)&sbits, sizeof(uint32_t) * (size_t)N * s_ldw);
+    if (e == hipSuccess) e = hipMalloc((void **)&sbitsT, sizeof(uint32_t) * (size_t)std::max<long>(32 * s_kw, 1) * s_ldwT);
+    if (e != hipSuccess) {
+      drop_sign();
+      return fail(KMT_ERR_MEMORY, "cannot allocate the packed sign factor (N=%ld, K=%d): %s", N, K, hipGetErrorString(e));
+    }
+    s_split1 = sign_splits(s_kw, N, &s_per1);
+    s_split2 = sign_splits(s_nw, std::max<long>(K, 1), &s_per2);
+    HIP_TRY(hipMalloc((void **)&s_part1, sizeof(double) * 2 * (size_t)s_split1 * 32 * std::max<long>(s_kw, 1)));
+    HIP_TRY(hipMalloc((void **)&s_part2, sizeof(double) * 2 * (size_t)s_split2 * 32 * s_nw));
+    HIP_TRY(hipMalloc((void **)&s_G, sizeof(double) * 2 * 32 * (size_t)std::max<long>(s_kw, 1)));
+    const size_t words = (size_t)N * s_ldw;
+    gen_sign_bits_kernel<<<(unsigned)((words + kBlock - 1) / kBlock), kBlock, 0, stream>>>(sbits, N, K, s_ldw, seed);
+    HIP_TRY(hipGetLastError());
+    if (K > 0) {
+      const size_t waves = (size_t)((N + 63) / 64) * s_kw;
+      transpose_sign_bits_kernel<<<(unsigned)((waves * kWave + kBlock - 1) / kBlock), kBlock, 0, stream>>>(
+          sbits, N, s_ldw, s_kw, sbitsT, s_ldwT);
+      HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipStreamSynchronize(stream));
+    sign = 1;
+    return KMT_OK;
+  }
+
+  template <class S> int gen_sign_rows(R inv_scale) {
+    constexpr int ROWS = 8;
+    const long chunk = 4096;
+    for (long r0 = 0; r0 < N; r0 += chunk) {
+      const long nr = std::min(chunk, N - r0);
+      dim3 grid((unsigned)((N + kBlock - 1) / kBlock), (unsigned)((nr + ROWS - 1) / ROWS));
+      gen_adj_sign_kernel<R, S, ROWS><<<grid, kBlock, 0, stream>>>(sbits, s_ldw, w, K, N, r0, nr,
+                                                                   (S *)A + (size_t)r0 * lda, lda, inv_scale);
+      HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipStreamSynchronize(stream));
+    return KMT_OK;
+  }
+
   template <class S> int gen_rows(R inv_scale) {
     const long chunk = 4096;
     for (long r0 = 0; r0 < N; r0 += chunk) {
@@ -496,11 +592,22 @@ template <class R> struct Engine : EngineBase {
   }
 
   int build_adjacency(int stg) override {
-    if (!X) return fail(KMT_ERR_STATE, "kmt_build_adjacency needs kmt_set_factor first");
+    if (!X && !sign) return fail(KMT_ERR_STATE, "kmt_build_adjacency needs kmt_set_factor or kmt_set_factor_sign first");
+    if (sign && is_sym(stg)) return fail(KMT_ERR_ARG, "symmetric storage is not built from a packed sign factor");
     HIP_TRY(hipStreamSynchronize(stream));
     drop_graphs();  // captured launches hold the old matrix scale
     KMT_TRY(ensure_A(stg));
     factored = 0;
+    if (sign) {
+      // |M_ij| <= sum_m |w_m| (X^2 = 1): scale the 16-bit forms so |stored| <= 1
+      double mx = 0;
+      for (int m = 0; m < K; ++m) mx += std::fabs((double)w_host[m]);
+      mscale_ = stg == KMT_ADJ_NATIVE ? 1.0 : pow2_at_least(mx);
+      const R inv = (R)(1.0 / mscale_);
+      if (stg == KMT_ADJ_NATIVE) return gen_sign_rows<R>((R)1);
+      if (stg == KMT_ADJ_BF16) return gen_sign_rows<bf16_t>(inv);
+      return gen_sign_rows<f16_t>(inv);
+    }
     if (stg == KMT_ADJ_NATIVE) {
       mscale_ = 1.0;
       return gen_rows<R>((R)1);
@@ -536,7 +643,9 @@ template <class R> struct Engine : EngineBase {
 
   int use_factored(int on) override {
     if (on < 0 || on > KMT_LOWRANK_REBUILD) return fail(KMT_ERR_ARG, "unknown low-rank mode %d", on);
-    if (on && !X) return fail(KMT_ERR_STATE, "-A needs kmt_set_factor first");
+    if (on && !X && !sign) return fail(KMT_ERR_STATE, "-A needs kmt_set_factor or kmt_set_factor_sign first");
+    if (on == KMT_LOWRANK_REBUILD && sign)
+      return fail(KMT_ERR_ARG, "register-rebuilt tiles need a real factor (kmt_set_factor), not a packed sign factor");
     if (on == KMT_LOWRANK_REBUILD) {
       if (K > kLrMaxRank) return fail(KMT_ERR_ARG, "register-rebuilt tiles support K <= %d (K=%d)", kLrMaxRank, K);
       if (!XT) {
@@ -607,7 +716,7 @@ template <class R> struct Engine : EngineBase {
     return v;
   }
 
-  int coupling_ready() const { return factored ? (X != nullptr) : (A != nullptr); }
+  int coupling_ready() const { return factored ? (X != nullptr || sign) : (A != nullptr); }
 
   // workgroups of one resident round of the balanced variant (<= N rows)
   template <class S, int ROWS, bool NT> int balanced_grid() {
@@ -672,7 +781,7 @@ template <class R> struct Engine : EngineBase {
     else launch_dense_kind<f16_t, KIND>(v);
   }
 
-  bool lr_fused() const { return K <= kLrMaxRank; }
+  bool lr_fused() const { return K <= kLrMaxRank && !sign; }
   // partial-sum buffers of the fused low-rank path (used alternately)
   double *lr_p(int which) const { return lr_part + (size_t)which * 2 * std::max(K, 1) * lr_grid; }
 
@@ -719,7 +828,28 @@ template <class R> struct Engine : EngineBase {
     }
   }
 
+  // packed sign factor: pass 1 over the sign bits, G, pass 2 over their
+  // transpose, sums + stage epilogue
+  template <int KIND> void launch_sign(const Vecs<R> &v) {
+    const int chk = v.check_state && KIND >= 2;
+    const long ldp1 = 32 * s_kw, ldp2 = 32 * s_nw;
+    if (K > 0) {
+      sign_sum_kernel<R, R><<<dim3((unsigned)((s_kw + kBlock - 1) / kBlock), (unsigned)s_split1), kBlock, 0, stream>>>(
+          sbits, s_ldw, s_kw, N, s_per1, v.cs_in, v.cs_in + ldv, 1, s_part1, ldp1, st, chk);
+      sign_gram_kernel<R><<<(unsigned)((ldp1 + kBlock - 1) / kBlock), kBlock, 0, stream>>>(s_part1, s_split1, ldp1, w,
+                                                                                             K, s_G, st, chk);
+      sign_sum_kernel<R, double><<<dim3((unsigned)((s_nw + kBlock - 1) / kBlock), (unsigned)s_split2), kBlock, 0,
+                                   stream>>>(sbitsT, s_ldwT, s_nw, K, s_per2, s_G, s_G + 1, 2, s_part2, ldp2, st, chk);
+    }
+    sign_stage_kernel<R, KIND><<<(unsigned)((N + kBlock - 1) / kBlock), kBlock, 0, stream>>>(
+        s_part2, K > 0 ? s_split2 : 0, ldp2, v);
+  }
+
   template <int KIND> void launch_factored(const Vecs<R> &v) {
+    if (sign) {
+      launch_sign<KIND>(v);
+      return;
+    }
     if (lr_fused()) {
       launch_gram(v.cs_in, 0, v.check_state && KIND >= 2);
       launch_lr_stage<KIND, false>(v, 0);

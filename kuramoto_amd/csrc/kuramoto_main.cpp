@@ -395,14 +395,20 @@ int main(int argc, char *argv[]) {
   }
   CALL(kmt_set_omega(g_ctx, omegas.data()));
 
-  // adjacency, kuramoto.cu:557-581
-  std::vector<double> Xd((size_t)N * K);
-  CALL(kmt_factor((long)N, (int)K, (uint32_t)seed, gauss, Xd.data()));
-  std::vector<R> X(Xd.begin(), Xd.end()), w(K);
-  Xd.clear();
-  Xd.shrink_to_fit();
-  CALL(kmt_factor_weights(PREC, (long)N, (int)K, gauss, w.data()));
-  CALL(kmt_set_factor(g_ctx, X.data(), w.data(), (int)K));
+  // adjacency, kuramoto.cu:557-581.  The volcano factor of rank K > 16 is
+  // drawn on the device and kept as packed signs (the same signs as the host
+  // draw); small ranks and Gaussian factors go through a real N x K factor.
+  if (!gauss && K > 16) {
+    CALL(kmt_set_factor_sign(g_ctx, (int)K, (uint32_t)seed));
+  } else {
+    std::vector<double> Xd((size_t)N * K);
+    CALL(kmt_factor((long)N, (int)K, (uint32_t)seed, gauss, Xd.data()));
+    std::vector<R> X(Xd.begin(), Xd.end()), w(K);
+    Xd.clear();
+    Xd.shrink_to_fit();
+    CALL(kmt_factor_weights(PREC, (long)N, (int)K, gauss, w.data()));
+    CALL(kmt_set_factor(g_ctx, X.data(), w.data(), (int)K));
+  }
   if (!A && reload && (in = std::fopen(fname("adj.dat").c_str(), "r"))) {
     std::printf("Using adjacency from file\n");
     std::fprintf(out, "Using adjacency from file\n");

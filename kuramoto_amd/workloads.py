@@ -44,6 +44,13 @@ PLAIN = {
     # stage (dense semantics, VALU-bound)
     "fp32-n200000-rebuild": dict(N=200000, K=5, seed=3, prec="f32", normal=False, init=0.0, c1=3.0, atol=1e-6,
                                  t1=20.0, dt=0.5, storage="rebuild"),
+    # the reference's published high-rank -A runs (volcano.ipynb:55-60 and
+    # :319-324): the volcano factor X = +-1 with K = N, drawn on the device
+    # and kept as packed signs (1 bit per entry), w_m = (-1)^m / N
+    "fp32-n10000-k10000-signs": dict(N=10000, K=10000, seed=1, prec="f32", normal=True, init=0.0, c1=1.75,
+                                     atol=1e-6, t1=100.0, dt=0.01, storage="signs"),
+    "fp32-n100000-k100000-signs": dict(N=100000, K=100000, seed=1, prec="f32", normal=True, init=0.0, c1=3.0,
+                                       atol=1e-6, t1=20.0, dt=0.02, storage="signs"),
     # the same N=200000 network with A stored explicitly in FP16 (80 GB)
     "fp32-n200000-f16": dict(N=200000, K=5, seed=3, prec="f32", normal=False, init=0.0, c1=3.0, atol=1e-6,
                              t1=20.0, dt=0.5, storage="f16"),

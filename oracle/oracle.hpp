@@ -242,15 +242,29 @@ template <class R> struct Model {
     if (adj.kind == 2) {
       const R *X = (const R *)adj.X, *w = (const R *)adj.w;
       int K = adj.K;
+      /* G_m = w_m sum_n X_nm (cos_n, sin_n), each sum in n order; a thread
+       * owns a strip of m and walks the rows of X (contiguous reads) */
       std::vector<double> gc(K, 0.0), gs(K, 0.0);
-      for (int m = 0; m < K; ++m) {
-        double ac = 0, as = 0;
+#pragma omp parallel
+      {
+        int nt = 1, tid = 0;
+#ifdef _OPENMP
+        nt = omp_get_num_threads();
+        tid = omp_get_thread_num();
+#endif
+        const int m0 = (int)((long)K * tid / nt), m1 = (int)((long)K * (tid + 1) / nt);
         for (long n = 0; n < N; ++n) {
-          ac += (double)X[n * K + m] * (double)cosv[n];
-          as += (double)X[n * K + m] * (double)sinv[n];
+          const double c = (double)cosv[n], sn = (double)sinv[n];
+          const R *Xn = X + n * K;
+          for (int m = m0; m < m1; ++m) {
+            gc[m] += (double)Xn[m] * c;
+            gs[m] += (double)Xn[m] * sn;
+          }
         }
-        gc[m] = (double)w[m] * ac;
-        gs[m] = (double)w[m] * as;
+        for (int m = m0; m < m1; ++m) {
+          gc[m] *= (double)w[m];
+          gs[m] *= (double)w[m];
+        }
       }
 #pragma omp parallel for schedule(static)
       for (long n = 0; n < N; ++n) {

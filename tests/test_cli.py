@@ -158,6 +158,34 @@ def test_cli_lowrank_and_reload(tmp_path, km, orc):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_cli_high_rank_volcano(tmp_path, km, orc, prec):
+    """Rank K = 40 > 16: the programs draw the volcano signs on the device
+    (packed, kmt_set_factor_sign); the oracle's programs draw them on the
+    host.  Dense: adj.dat byte-identical to the oracle's; -A: every output
+    row within 1e-5 (FP32) / 1e-9 (FP64) of the oracle's -A run."""
+    prog = os.path.join(BIN, "kuramoto" if prec == "f32" else "kuramoto_64")
+    oprog = os.path.join(ORC, "kuramoto_oracle" if prec == "f32" else "kuramoto_oracle_64")
+    dt = np.float32 if prec == "f32" else np.float64
+    args = ["-N", "600", "-K", "40", "-c", "1.75", "-t", "1", "-d", "0.125", "-s", "5", "-n",
+            "-a", "1e-6" if prec == "f32" else "1e-9"]
+    print(f"case cli high-rank volcano {prec}", flush=True)
+    g, o = str(tmp_path / "g"), str(tmp_path / "o")
+    run(prog, args, g)
+    run(oprog, args, o)
+    assert open(g + "adj.dat", "rb").read() == open(o + "adj.dat", "rb").read()
+    ga, oa = str(tmp_path / "ga"), str(tmp_path / "oa")
+    run(prog, args + ["-A", "-D", "2"], ga)
+    run(oprog, args + ["-A", "-D", "2"], oa)
+    check_layout(ga, 600, 9, dt, dense=2, A=True)
+    thg = load(ga, "thetas.dat", dt).reshape(9, 600)
+    tho = load(oa, "thetas.dat", dt).reshape(9, 600)
+    tol = 1e-9 if prec == "f64" else 1e-5
+    assert np.max(np.abs(thg - tho) / np.maximum(1, np.abs(tho))) < tol
+    assert np.max(np.abs(load(ga, "order.dat", dt) - load(oa, "order.dat", dt))) < 1e-3
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["f32", "f64"])
 def test_cli_rebuild_equals_dense(tmp_path, km, orc, prec):
     """`-A --rebuild` (adjacency tiles rebuilt in registers from the factor,
     no adj.dat) gives the dense run's files bit for bit, and follows the
