@@ -323,8 +323,12 @@ int kmt_batch_set_vector_split(kmt_batch *batch, int hi_lo);
 int kmt_batch_set_proposer(kmt_batch *batch, const uint16_t *weights_f16, int n_params, int enabled);
 int kmt_proposer_param_count(void);
 /* attempt log for training the proposer: capacity records of 14 floats
- * {8 features, h used, h proposed, error norm, accepted, instance,
- *  1 if h came from the proposal made after an accepted step} */
+ * {8 features, h used, the standard controller's proposal behind it, error
+ *  norm, accepted, instance, 1 if h came from the proposal made after an
+ *  accepted step}, returned sorted by (instance, attempt) so that a run
+ * reads back the same log every time; reading a log that overflowed its
+ * capacity fails (the kept subset would depend on the order the
+ * instances' controllers ran in). */
 int kmt_batch_set_attempt_log(kmt_batch *batch, int capacity);
 int kmt_batch_read_attempt_log(kmt_batch *batch, int max_n, float *records, int *n);
 

@@ -31,15 +31,13 @@
 #include "../../include/kuramoto_amd.h"
 #include "kmt_common.hpp"
 #include "kmt_kernels.hpp"
+#include "kmt_mlp.hpp"
 
 namespace kmt {
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
-
-constexpr int kMaxFeat = 8;    // proposer inputs
-constexpr int kHidden = 64;    // proposer hidden width
 
 // Per-instance device state.
 struct BInst {
@@ -53,9 +51,10 @@ struct BInst {
   long long proposer_accepts, proposer_rejects;
   double norm;
   float logh[3], logerr[3];  // history for the proposer (most recent first)
-  float r_now, r_prev;       // order parameter at the last two accepted steps
-  float t_prev;
-  int hist_n;
+  float r_now, r_prev;       // order parameter at the last two emitted outputs
+  float tr_now, tr_prev;     // and their output times
+  float h_std;               // the standard controller's proposal behind the current h
+  int hist_n, r_n;           // accepted steps (up to 3) and outputs (up to 2) in the history
   int vrow;             // shared mode: row of cos in V^T and in the product (sin: vrow + half); -1 = none
 };
 
@@ -536,69 +535,41 @@ __global__ __launch_bounds__(kBlock) void bepilogue_kernel(const BVecs a) {
 }
 
 // ---------------------------------------------------------------------
-// Learned step-size proposer: MLP 8 -> 64 -> 64 -> 1 (tanh, tanh, linear),
-// FP16 weights, FP32 accumulation.  Output = log(h_next / h_standard), i.e.
-// a correction of the standard controller's proposal, clamped to [-0.7, 0.7].
+// Learned step-size proposer: the network of kmt_mlp.hpp, whose output is a
+// correction log(h_next / h_standard) of the standard controller's
+// proposal, clamped to [-0.7, 0.7].  Evaluated by wave 0 of the
+// controller's block (one lane per hidden unit), with the same function as
+// the training kernels' forward pass.
 // ---------------------------------------------------------------------
-struct MlpW {
-  const uint16_t *w;  // FP16 parameters, packed: W1[64][8], b1[64], W2[64][64], b2[64], W3[64], b3
-  int enabled;
-};
-constexpr int kMlpParams = kHidden * kMaxFeat + kHidden + kHidden * kHidden + kHidden + kHidden + 1;
+constexpr float kProposerClamp = 0.7f;
 
-__device__ __forceinline__ float h2f(uint16_t u) { return (float)__builtin_bit_cast(_Float16, u); }
-
-// one thread evaluates the network (called by thread 0 of the controller)
-__device__ float mlp_eval(const uint16_t *w, const float *x) {
-  const uint16_t *W1 = w, *b1 = W1 + kHidden * kMaxFeat, *W2 = b1 + kHidden, *b2 = W2 + kHidden * kHidden,
-                 *W3 = b2 + kHidden, *b3 = W3 + kHidden;
-  float h1[kHidden];
-  for (int i = 0; i < kHidden; ++i) {
-    float a = h2f(b1[i]);
-    for (int j = 0; j < kMaxFeat; ++j) a = fmaf(h2f(W1[i * kMaxFeat + j]), x[j], a);
-    h1[i] = tanhf(a);
-  }
-  float out = h2f(b3[0]);
-  for (int i = 0; i < kHidden; ++i) {
-    float a = h2f(b2[i]);
-    for (int j = 0; j < kHidden; ++j) a = fmaf(h2f(W2[i * kHidden + j]), h1[j], a);
-    out = fmaf(h2f(W3[i]), tanhf(a), out);
-  }
-  return fminf(fmaxf(out, -0.7f), 0.7f);
-}
-
-__device__ void proposer_features(const BInst *in, float h_std, float *x) {
-  // last 3 log h (relative to the standard proposal), last 3 log error ratios,
-  // r and dr/dt
+// Inputs: the last 3 log h relative to the standard proposal, the last 3 log
+// error norms, r at the last output and its slope over the last two outputs.
+__device__ void proposer_features(const BInst *in, float h_std, float (&x)[kIn]) {
   const float lh = logf(h_std);
   for (int q = 0; q < 3; ++q) x[q] = in->hist_n > q ? in->logh[q] - lh : 0.f;
   for (int q = 0; q < 3; ++q) x[3 + q] = in->hist_n > q ? in->logerr[q] : 0.f;
   x[6] = in->r_now;
-  const float dtp = in->t - in->t_prev;
-  x[7] = (in->hist_n > 1 && dtp > 0.f) ? fminf(fmaxf((in->r_now - in->r_prev) / dtp, -10.f), 10.f) : 0.f;
+  const float dtr = in->tr_now - in->tr_prev;
+  x[7] = (in->r_n > 1 && dtr > 0.f) ? fminf(fmaxf((in->r_now - in->r_prev) / dtr, -10.f), 10.f) : 0.f;
 }
 
 struct LogRec {   // one accepted/rejected attempt for training the proposer
-  float x[kMaxFeat];
+  float x[kIn];
   float h_used, h_std, err_norm;
   int accepted, b;
   int after_accept;  // h came from the proposal made after an accepted step
+  int attempt;       // the instance's attempt number (orders the log)
 };
 
-// Controller per instance (dp45.cu:117-138), block b.  With the proposer on,
-// the next h after an accepted step is h_std * exp(mlp(x)); after a
-// rejection the standard controller is used (fallback).
-__global__ __launch_bounds__(kBlock) void bcontrol_kernel(BInst *inst, const double *err_part, int nblk, float t1,
-                                                          int fixed, const float *t_eval, int n_eval, MlpW mlp,
-                                                          LogRec *log, int *log_count, int log_cap) {
-  const int b = blockIdx.x;
-  BInst *in = inst + b;
-  if (!in->active || in->commit_pending) return;
-  __shared__ double sh[kBlock];
-  double s = 0.0;
-  for (int q = threadIdx.x; q < nblk; q += kBlock) s += err_part[(size_t)b * nblk + q];
-  const double sum = block_sum(s, sh);
-  if (threadIdx.x != 0) return;
+// The controller's decision for one instance (dp45.cu:117-138), run by one
+// thread: accept or reject the attempt, the standard controller's next step,
+// the proposer's history, the outputs an accepted step covers.  Returns 1
+// when the proposer is to replace the standard next step (x: its inputs,
+// h_std: the standard step it corrects).
+__device__ int control_decide(BInst *in, double sum, int b, int fixed, const float *t_eval, int n_eval,
+                              bool propose, LogRec *log, int *log_count, int log_cap, float (&x)[kIn],
+                              float &h_std) {
   in->attempts += 1;
   in->rhs_evals += 6;
   const float t = in->t, h = in->h;
@@ -627,18 +598,20 @@ __global__ __launch_bounds__(kBlock) void bcontrol_kernel(BInst *inst, const dou
       }
     }
   }
-  // training log of every attempt (features as seen before this attempt)
+  // training log of every attempt (inputs as seen before this attempt,
+  // relative to the step it used)
   if (log && !fixed) {
     const int slot = atomicAdd(log_count, 1);
     if (slot < log_cap) {
       LogRec &L = log[slot];
       proposer_features(in, h, L.x);
       L.h_used = h;
-      L.h_std = h;
+      L.h_std = in->h_std;
       L.err_norm = norm;
       L.accepted = accept;
       L.b = b;
       L.after_accept = in->last_accept;
+      L.attempt = (int)in->attempts;
     }
   }
   if (in->proposed) {
@@ -646,13 +619,13 @@ __global__ __launch_bounds__(kBlock) void bcontrol_kernel(BInst *inst, const dou
     else in->proposer_rejects += 1;
   }
   in->last_accept = accept;
+  in->h = hstd;
+  in->h_std = hstd;
+  in->proposed = 0;
   if (!accept) {
     in->rejected += 1;
-    in->h = hstd;
-    in->proposed = 0;
-    return;
+    return 0;
   }
-  // history for the proposer
   in->logh[2] = in->logh[1];
   in->logh[1] = in->logh[0];
   in->logh[0] = logf(h);
@@ -661,24 +634,55 @@ __global__ __launch_bounds__(kBlock) void bcontrol_kernel(BInst *inst, const dou
   in->logerr[0] = logf(fmaxf(norm, 1e-12f));
   in->hist_n = min(in->hist_n + 1, 3);
   const float tn = t + h;
-  in->t_prev = t;
   in->t_last = t;
   in->t = tn;
-  float hnext = hstd;
-  in->proposed = 0;
-  if (mlp.enabled && !fixed && in->hist_n >= 3) {
-    float x[kMaxFeat];
-    proposer_features(in, hstd, x);
-    hnext = hstd * expf(mlp_eval(mlp.w, x));
-    in->proposed = 1;
-  }
-  in->h = hnext;
   in->accepted += 1;
   int j = in->eval_i;
   while (j < n_eval && tn >= t_eval[j]) ++j;
   in->pend_lo = in->eval_i;
   in->pend_hi = j;
   in->commit_pending = 1;
+  if (!propose || fixed || in->hist_n < 3) return 0;
+  proposer_features(in, hstd, x);
+  h_std = hstd;
+  return 1;
+}
+
+// Controller per instance, block b: the error norm from the stage-7
+// partials (fixed-order block sum), the decision by thread 0, and with the
+// proposer on (PROPOSE) the next h after an accepted step = h_std
+// exp(clamp(mlp(x))), the network evaluated by wave 0; after a rejection
+// the standard controller's step is kept (fallback).
+template <bool PROPOSE>
+__global__ __launch_bounds__(kBlock) void bcontrol_kernel(BInst *inst, const double *err_part, int nblk, float t1,
+                                                          int fixed, const float *t_eval, int n_eval,
+                                                          const uint16_t *mlp_w, LogRec *log, int *log_count,
+                                                          int log_cap) {
+  const int b = blockIdx.x;
+  BInst *in = inst + b;
+  if (!in->active || in->commit_pending) return;
+  __shared__ double sh[kBlock];
+  double s = 0.0;
+  for (int q = threadIdx.x; q < nblk; q += kBlock) s += err_part[(size_t)b * nblk + q];
+  const double sum = block_sum(s, sh);
+  if (threadIdx.x >= kWave) return;
+  float x[kIn] = {};
+  float hstd = 0.f;
+  int use = 0;
+  if (threadIdx.x == 0)
+    use = control_decide(in, sum, b, fixed, t_eval, n_eval, PROPOSE, log, log_count, log_cap, x, hstd);
+  if constexpr (PROPOSE) {
+    if (__shfl(use, 0, kWave)) {
+#pragma unroll
+      for (int j = 0; j < kIn; ++j) x[j] = __shfl(x[j], 0, kWave);
+      float h1, h2;
+      const float o = mlp_wave_forward(mlp_w, x, h1, h2);
+      if (threadIdx.x == 0) {
+        in->h = hstd * expf(fminf(fmaxf(o, -kProposerClamp), kProposerClamp));
+        in->proposed = 1;
+      }
+    }
+  }
 }
 
 // Dense output (order parameter only), commit, next stage-2 vector.
@@ -757,10 +761,9 @@ __global__ __launch_bounds__(kBlock) void bemit_kernel(const BEmit a) {
 }
 
 // r for the emitted rows, ring bookkeeping, end of run per instance; also
-// the order parameter at the accepted step for the proposer's features
-// (the last emitted output's r).
+// the last two emitted outputs' r and times for the proposer's inputs.
 __global__ __launch_bounds__(kBlock) void bfinalize_kernel(BInst *inst, const double *ord_part, int nblk, int ring_cap,
-                                                           float t1, float *r_ring) {
+                                                           float t1, const float *t_eval, float *r_ring) {
   const int b = blockIdx.x;
   BInst *in = inst + b;
   if (!in->active || !in->commit_pending) return;
@@ -769,7 +772,7 @@ __global__ __launch_bounds__(kBlock) void bfinalize_kernel(BInst *inst, const do
   const int room = ring_cap - in->ring_count;
   const int n_emit = pend < room ? pend : room;
   const int base = in->ring_count;
-  float r_last = in->r_now;
+  float r_last = in->r_now, r_before = in->r_prev;
   for (int q = 0; q < n_emit; ++q) {
     const int slot = base + q;
     double sx = 0, sy = 0;
@@ -781,13 +784,18 @@ __global__ __launch_bounds__(kBlock) void bfinalize_kernel(BInst *inst, const do
     const double X = block_sum(sx, sh), Y = block_sum(sy, sh);
     const float r = order_value<float>(X, Y, in->n);
     if (threadIdx.x == 0) r_ring[(size_t)b * ring_cap + slot] = r;
+    r_before = r_last;
     r_last = r;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     if (n_emit > 0) {
-      in->r_prev = in->r_now;
+      const int last = in->pend_lo + n_emit - 1;
+      in->tr_prev = n_emit > 1 ? t_eval[last - 1] : in->tr_now;
+      in->tr_now = t_eval[last];
+      in->r_prev = r_before;
       in->r_now = r_last;
+      in->r_n = min(in->r_n + n_emit, 2);
     }
     in->ring_count = base + n_emit;
     in->pend_lo += n_emit;
@@ -1171,12 +1179,6 @@ struct kmt_batch {
     if (storage == KMT_ADJ_F16) bemit_kernel<f16_t><<<grid, kBlock, 0, stream>>>(bemit_args());
     else bemit_kernel<bf16_t><<<grid, kBlock, 0, stream>>>(bemit_args());
   }
-  MlpW mlpw() const {
-    MlpW m;
-    m.w = mlp_w;
-    m.enabled = mlp_on && mlp_w;
-    return m;
-  }
 
   void launch_attempt(std::vector<std::pair<hipEvent_t, hipEvent_t>> *ev = nullptr) {
     for (int s = 2; s <= 7; ++s) {
@@ -1191,9 +1193,14 @@ struct kmt_batch {
       }
       epilogue(s, bvecs(in, out));
     }
-    bcontrol_kernel<<<B, kBlock, 0, stream>>>(inst, err_part, nblk, t1, fixed, t_eval, n_eval, mlpw(), log, log_count, log_cap);
+    if (mlp_on && mlp_w)
+      bcontrol_kernel<true><<<B, kBlock, 0, stream>>>(inst, err_part, nblk, t1, fixed, t_eval, n_eval, mlp_w, log,
+                                                      log_count, log_cap);
+    else
+      bcontrol_kernel<false><<<B, kBlock, 0, stream>>>(inst, err_part, nblk, t1, fixed, t_eval, n_eval, nullptr, log,
+                                                       log_count, log_cap);
     emit();
-    bfinalize_kernel<<<B, kBlock, 0, stream>>>(inst, ord_part, nblk, ring_cap, t1, r_ring);
+    bfinalize_kernel<<<B, kBlock, 0, stream>>>(inst, ord_part, nblk, ring_cap, t1, t_eval, r_ring);
   }
 
   int build_graphs() {
@@ -1260,7 +1267,7 @@ struct kmt_batch {
       const int n = in.n;
       std::memset(&in, 0, sizeof(BInst));
       in.t = t0; in.h = h0; in.c1 = c1; in.mscale = ms; in.n = n;
-      in.t_last = t0; in.t_prev = t0;
+      in.t_last = t0; in.h_std = h0;
       in.active = t0 < t1 ? 1 : 0;
       in.rhs_evals = 1;
       in.last_accept = 1;
@@ -1505,9 +1512,9 @@ struct kmt_batch {
 
   int set_proposer(const uint16_t *w, int nparams, int on) {
     if (w) {
-      if (nparams != kMlpParams) return kmt_fail(KMT_ERR_ARG, "proposer needs %d parameters (got %d)", kMlpParams, nparams);
-      if (!mlp_w) KMT_TRY(alloc(&mlp_w, (size_t)kMlpParams));
-      HIP_TRY(hipMemcpyAsync(mlp_w, w, sizeof(uint16_t) * kMlpParams, hipMemcpyHostToDevice, stream));
+      if (nparams != kNParams) return kmt_fail(KMT_ERR_ARG, "proposer needs %d parameters (got %d)", kNParams, nparams);
+      if (!mlp_w) KMT_TRY(alloc(&mlp_w, (size_t)kNParams));
+      HIP_TRY(hipMemcpyAsync(mlp_w, w, sizeof(uint16_t) * kNParams, hipMemcpyHostToDevice, stream));
       HIP_TRY(hipStreamSynchronize(stream));
     }
     if (on && !mlp_w) return kmt_fail(KMT_ERR_STATE, "no proposer weights");
@@ -1534,16 +1541,24 @@ struct kmt_batch {
     int cnt = 0;
     HIP_TRY(hipMemcpyAsync(&cnt, log_count, sizeof(int), hipMemcpyDeviceToHost, stream));
     HIP_TRY(hipStreamSynchronize(stream));
-    cnt = std::min(cnt, log_cap);
-    const int n = std::min(cnt, std::max(max_n, 0));
-    std::vector<LogRec> buf(std::max(n, 1));
-    if (n) {
-      HIP_TRY(hipMemcpyAsync(buf.data(), log, sizeof(LogRec) * n, hipMemcpyDeviceToHost, stream));
+    // The slots are taken in whatever order the instances' controllers run:
+    // the records are returned sorted by (instance, attempt), the same on
+    // every run.  Past the capacity the kept subset would depend on that
+    // order, so an overflowed log is an error.
+    if (cnt > log_cap)
+      return kmt_fail(KMT_ERR_STATE, "attempt log overflowed (%d attempts, capacity %d)", cnt, log_cap);
+    std::vector<LogRec> buf(std::max(cnt, 1));
+    if (cnt) {
+      HIP_TRY(hipMemcpyAsync(buf.data(), log, sizeof(LogRec) * cnt, hipMemcpyDeviceToHost, stream));
       HIP_TRY(hipStreamSynchronize(stream));
     }
+    std::sort(buf.begin(), buf.begin() + cnt, [](const LogRec &u, const LogRec &v) {
+      return u.b != v.b ? u.b < v.b : u.attempt < v.attempt;
+    });
+    const int n = std::min(cnt, std::max(max_n, 0));
     for (int q = 0; q < n; ++q) {
       float *o = rec + (size_t)q * 14;
-      for (int j = 0; j < kMaxFeat; ++j) o[j] = buf[q].x[j];
+      for (int j = 0; j < kIn; ++j) o[j] = buf[q].x[j];
       o[8] = buf[q].h_used; o[9] = buf[q].h_std; o[10] = buf[q].err_norm; o[11] = (float)buf[q].accepted;
       o[12] = (float)buf[q].b; o[13] = (float)buf[q].after_accept;
     }
@@ -1665,6 +1680,6 @@ int kmt_batch_read_attempt_log(kmt_batch *bp, int max_n, float *records, int *n)
   if (!records) return kmt_fail(KMT_ERR_ARG, "records is NULL");
   return bp->read_log(max_n, records, n);
 }
-int kmt_proposer_param_count(void) { return kMlpParams; }
+int kmt_proposer_param_count(void) { return kNParams; }
 
 }  // extern "C"

@@ -1,14 +1,14 @@
-// Learned step-size proposer: training and inference kernels of the MLP
-// 8 -> 64 -> 64 -> 1 (tanh, tanh, linear) used by the batched controller.
-// FP16 weights for inference, FP32 master weights for the optimizer, FP32
-// accumulation everywhere.  No reference counterpart (an extension of the path).
+// Learned step-size proposer: training kernels of the network in
+// kmt_mlp.hpp (8 -> 64 -> 64 -> 1, tanh, tanh, linear).  FP16 weights for the
+// forward pass, FP32 master weights for the optimizer, FP32 accumulation
+// everywhere.  No reference counterpart (an extension of the path).
 //
-// Parameter packing (4801 values): W1[64][8], b1[64], W2[64][64], b2[64],
-// W3[64], b3.  Loss: mean squared error (1/S) sum_s (out_s - y_s)^2.
+// Loss: mean squared error (1/S) sum_s (out_s - y_s)^2.
 //
-//   mlp_fwd_bwd_kernel  one wave per sample, one lane per hidden unit:
-//                       forward, output delta, back-propagated deltas;
-//                       stores h1, h2, d1, d2 (S x 64) and d3 (S)
+//   mlp_fwd_bwd_kernel  one wave per sample, one lane per hidden unit: the
+//                       forward pass the batched controller runs
+//                       (mlp_wave_forward), output delta, back-propagated
+//                       deltas; stores h1, h2, d1, d2 (S x 64) and d3 (S)
 //   mlp_grad_kernel     one thread per parameter, fixed-order sum over the
 //                       samples (deterministic)
 //   mlp_adam_kernel     Adam update of the FP32 master weights, FP16 copy
@@ -21,20 +21,14 @@
 #include "../../include/kuramoto_amd.h"
 #include "kmt_common.hpp"
 #include "kmt_kernels.hpp"
+#include "kmt_mlp.hpp"
 #include "kmt_rng.h"
 
 namespace kmt {
 
-constexpr int kIn = 8, kHid = 64;
-constexpr int kOffW1 = 0, kOffB1 = kOffW1 + kHid * kIn, kOffW2 = kOffB1 + kHid, kOffB2 = kOffW2 + kHid * kHid,
-              kOffW3 = kOffB2 + kHid, kOffB3 = kOffW3 + kHid, kNParams = kOffB3 + 1;
-
-__device__ __forceinline__ float hf(uint16_t u) { return (float)__builtin_bit_cast(_Float16, u); }
-
 __global__ __launch_bounds__(kBlock) void mlp_fwd_bwd_kernel(const uint16_t *w, const float *X, const float *y, int S,
                                                              float *H1, float *H2, float *D1, float *D2, float *D3,
                                                              float *out, double *loss_part) {
-  __shared__ float h1s[kBlock / kWave][kHid], h2s[kBlock / kWave][kHid], d2s[kBlock / kWave][kHid];
   __shared__ double lsum[kBlock / kWave];
   const int wv = threadIdx.x / kWave, i = threadIdx.x % kWave;
   const int s = blockIdx.x * (kBlock / kWave) + wv;
@@ -42,28 +36,14 @@ __global__ __launch_bounds__(kBlock) void mlp_fwd_bwd_kernel(const uint16_t *w, 
   float xs[kIn];
 #pragma unroll
   for (int j = 0; j < kIn; ++j) xs[j] = valid ? X[(size_t)s * kIn + j] : 0.f;
-  // layer 1
-  float a = hf(w[kOffB1 + i]);
-#pragma unroll
-  for (int j = 0; j < kIn; ++j) a = fmaf(hf(w[kOffW1 + i * kIn + j]), xs[j], a);
-  const float h1 = tanhf(a);
-  h1s[wv][i] = h1;
-  __syncthreads();
-  // layer 2
-  a = hf(w[kOffB2 + i]);
-  for (int j = 0; j < kHid; ++j) a = fmaf(hf(w[kOffW2 + i * kHid + j]), h1s[wv][j], a);
-  const float h2 = tanhf(a);
-  h2s[wv][i] = h2;
-  // output (wave reduction of W3 . h2)
-  float o = wave_sum(hf(w[kOffW3 + i]) * h2) + hf(w[kOffB3]);
+  float h1, h2;
+  const float o = mlp_wave_forward(w, xs, h1, h2);
   const float ys = valid ? y[s] : 0.f;
   const float d3 = valid ? 2.f * (o - ys) / (float)S : 0.f;
-  // back-propagation
+  // back-propagation: d2_i = d3 W3_i (1 - h2_i^2), d1_i = (sum_k W2_ki d2_k) (1 - h1_i^2)
   const float d2 = d3 * hf(w[kOffW3 + i]) * (1.f - h2 * h2);
-  d2s[wv][i] = d2;
-  __syncthreads();
   float g = 0.f;
-  for (int k = 0; k < kHid; ++k) g = fmaf(hf(w[kOffW2 + k * kHid + i]), d2s[wv][k], g);
+  for (int k = 0; k < kHid; ++k) g = fmaf(hf(w[kOffW2 + k * kHid + i]), __shfl(d2, k, kWave), g);
   const float d1 = g * (1.f - h1 * h1);
   if (valid) {
     H1[(size_t)s * kHid + i] = h1;

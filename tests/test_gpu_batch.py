@@ -156,19 +156,19 @@ def test_attempt_log_and_proposer_fallback(km):
         assert np.max(np.abs(runs["standard"]["r"][b] - runs["proposer"]["r"][b])) < 1e-3
 
 
-@pytest.mark.parametrize("storage,B", [("bf16", 5), ("f16", 5), ("bf16", 40)])
+@pytest.mark.parametrize("storage,B", [("bf16", 5), ("f16", 5), ("bf16", 40), ("bf16", 64)])
 def test_shared_workgroup_variants_agree(km, storage, B):
     """The six workgroup shapes of the MFMA product sum every output in the
     same order (chunk, k-step, hi then lo) within a split-K slab: at a fixed
     split they give bit-identical runs (B = 5: one 32-row tile of V^T,
-    B = 40: three)."""
+    B = 40: three, B = 64: four)."""
     N = 1500
     X = km.factor(N, 5, 9, gaussian=True).astype(np.float32)
     w = km.factor_weights(km.F32, N, 5, gaussian=True)
     te = _grid(1.0, 0.25)
     out = []
     for var in range(6):
-        print(f"case shared variant {var} {storage}", flush=True)
+        print(f"case shared variant {var} {storage} B={B}", flush=True)
         with km.Batch([N] * B, shared=True, storage=storage) as bt:
             for b in range(B):
                 th, om = km.draws(km.F32, N, 40 + b, normal=True, init=6.283185307)
