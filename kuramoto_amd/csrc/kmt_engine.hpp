@@ -92,12 +92,17 @@ template <class R> struct Engine : EngineBase {
   double *lr_part = nullptr, *lr_G = nullptr;
   int lr_nblk = 0, lr_grid = 0;
   // packed +-1 factor (kmt_set_factor_sign): sign bits [N][s_ldw] and their
-  // transpose [32 s_kw][s_ldwT]; pass-1 / pass-2 partial sums and G
+  // transpose [32 s_kw][s_ldwT]; per pass (1: rows m over k = n, 2: rows n
+  // over k = m) the B fragments of the digits and the int32 partial sums
   int sign = 0;
   uint32_t *sbits = nullptr, *sbitsT = nullptr;
-  long s_kw = 0, s_ldw = 0, s_nw = 0, s_ldwT = 0, s_per1 = 1, s_per2 = 1;
-  int s_split1 = 1, s_split2 = 1;
-  double *s_part1 = nullptr, *s_part2 = nullptr, *s_G = nullptr;
+  long s_kw = 0, s_ldw = 0, s_nw = 0, s_ldwT = 0;
+  struct SignPass {
+    long ldp = 0, nsb = 0, sb_per = 1;
+    int nsplit = 1;
+    uint8_t *bfrag = nullptr;
+    int32_t *part = nullptr;
+  } s_pass[2];
   std::vector<R> X_host, w_host;
 
   // run
@@ -122,9 +127,8 @@ template <class R> struct Engine : EngineBase {
     R *vecs[] = {y, ylast, ynew, omega, kbuf, cs[0], cs[1], cs_scr, vec_scr, f_scr, pairs, k_scr, X, w, XT, t_eval, ring, r_ring, times};
     for (R *p : vecs)
       if (p) (void)hipFree(p);
-    double *dv[] = {err_part, ord_scr, lr_part, lr_G, ord_part, s_part1, s_part2, s_G};
-    if (sbits) (void)hipFree(sbits);
-    if (sbitsT) (void)hipFree(sbitsT);
+    double *dv[] = {err_part, ord_scr, lr_part, lr_G, ord_part};
+    drop_sign();
     for (double *p : dv)
       if (p) (void)hipFree(p);
     if (st) (void)hipFree(st);
@@ -497,31 +501,42 @@ template <class R> struct Engine : EngineBase {
   }
 
   void drop_sign() {
-    uint32_t *b[] = {sbits, sbitsT};
-    for (uint32_t *p : b)
-      if (p) (void)hipFree(p);
-    double *d[] = {s_part1, s_part2, s_G};
-    for (double *p : d)
+    void *b[] = {sbits, sbitsT, s_pass[0].bfrag, s_pass[0].part, s_pass[1].bfrag, s_pass[1].part};
+    for (void *p : b)
       if (p) (void)hipFree(p);
     sbits = sbitsT = nullptr;
-    s_part1 = s_part2 = s_G = nullptr;
+    s_pass[0] = s_pass[1] = SignPass();
     sign = 0;
   }
 
-  // splits of the outer rows so that a pass runs about 2048 waves (8 per
-  // CU): words / 256 workgroups of 4 waves per split, at least 32 rows each
-  static int sign_splits(long nwords, long n_outer, long *per) {
-    const long wg = (nwords + kBlock - 1) / kBlock;
-    long sp = std::max<long>(1, 512 / std::max<long>(wg, 1));
-    sp = std::min<long>(sp, std::max<long>(1, n_outer / 32));
-    *per = std::max<long>(1, (n_outer + sp - 1) / sp);
-    return (int)((n_outer + *per - 1) / *per);
+  // 16-row tiles per wave of the int8 passes
+  static constexpr int kSignNT = 4;
+  // Shape of a pass with `rows` output rows (+ the ones row) over kdim: row
+  // groups of 16 kSignNT rows x splits of the 512-wide super-blocks, about
+  // 2048 waves (8 per CU) with at least 2 super-blocks per split.
+  static SignPass sign_pass_shape(long rows, long kdim) {
+    SignPass p;
+    const long rg = (rows + 1 + 16 * kSignNT - 1) / (16 * kSignNT);
+    p.ldp = rg * 16 * kSignNT;
+    p.nsb = std::max<long>(1, (kdim + kSignSB - 1) / kSignSB);
+    long sp = std::max<long>(1, (2048 + rg - 1) / rg);
+    sp = std::min<long>(sp, std::max<long>(1, p.nsb / 2));
+    p.sb_per = (p.nsb + sp - 1) / sp;
+    p.nsplit = (int)((p.nsb + p.sb_per - 1) / p.sb_per);
+    return p;
+  }
+  int alloc_sign_pass(SignPass &p) {
+    HIP_TRY(hipMalloc((void **)&p.bfrag, (size_t)p.nsb * 8 * kWave * 16));
+    HIP_TRY(hipMalloc((void **)&p.part, sizeof(int32_t) * 16 * (size_t)p.nsplit * p.ldp));
+    return KMT_OK;
   }
 
   // The volcano factor X_nm = +-1 of the library's sign stream, packed, and
   // w_m = (-1)^m / N (makeu1 / makeu2 / m1 / m2, kuramoto.cu:62-143).
   int set_factor_sign(int k_, uint32_t seed) override {
     if (k_ < 0) return fail(KMT_ERR_ARG, "rank K must be >= 0");
+    if (N > kSignMaxDim || k_ > kSignMaxDim)
+      return fail(KMT_ERR_ARG, "the packed sign factor needs N, K <= %ld (N=%ld, K=%d)", kSignMaxDim, N, k_);
     HIP_TRY(hipStreamSynchronize(stream));
     drop_graphs();
     if (X) { (void)hipFree(X); X = nullptr; }
@@ -538,7 +553,7 @@ template <class R> struct Engine : EngineBase {
     HIP_TRY(hipMalloc((void **)&w, sizeof(R) * std::max(K, 1)));
     if (K > 0) HIP_TRY(hipMemcpyAsync(w, w_host.data(), sizeof(R) * K, hipMemcpyHostToDevice, stream));
     s_kw = (K + 31) / 32;
-    s_ldw = round_up(std::max<long>(s_kw, 1), 4);
+    s_ldw = round_up(std::max<long>(s_kw, 1), 16);  // whole super-blocks of the pass-2 rows
     s_nw = (N + 31) / 32;
     s_ldwT = round_up(s_nw, 64);
     hipError_t e = hipMalloc((void **)&sbits, sizeof(uint32_t) * (size_t)N * s_ldw);
@@ -547,11 +562,14 @@ template <class R> struct Engine : EngineBase {
       drop_sign();
       return fail(KMT_ERR_MEMORY, "cannot allocate the packed sign factor (N=%ld, K=%d): %s", N, K, hipGetErrorString(e));
     }
-    s_split1 = sign_splits(s_kw, N, &s_per1);
-    s_split2 = sign_splits(s_nw, std::max<long>(K, 1), &s_per2);
-    HIP_TRY(hipMalloc((void **)&s_part1, sizeof(double) * 2 * (size_t)s_split1 * 32 * std::max<long>(s_kw, 1)));
-    HIP_TRY(hipMalloc((void **)&s_part2, sizeof(double) * 2 * (size_t)s_split2 * 32 * s_nw));
-    HIP_TRY(hipMalloc((void **)&s_G, sizeof(double) * 2 * 32 * (size_t)std::max<long>(s_kw, 1)));
+    s_pass[0] = sign_pass_shape(K, N);
+    s_pass[1] = sign_pass_shape(N, K);
+    if (alloc_sign_pass(s_pass[0]) != KMT_OK || alloc_sign_pass(s_pass[1]) != KMT_OK) {
+      drop_sign();
+      return fail(KMT_ERR_MEMORY, "cannot allocate the sign passes' buffers (N=%ld, K=%d)", N, K);
+    }
+    // words of bitsT past N are never written by the transpose: zero them
+    HIP_TRY(hipMemsetAsync(sbitsT, 0, sizeof(uint32_t) * (size_t)std::max<long>(32 * s_kw, 1) * s_ldwT, stream));
     const size_t words = (size_t)N * s_ldw;
     gen_sign_bits_kernel<<<(unsigned)((words + kBlock - 1) / kBlock), kBlock, 0, stream>>>(sbits, N, K, s_ldw, seed);
     HIP_TRY(hipGetLastError());
@@ -828,21 +846,26 @@ template <class R> struct Engine : EngineBase {
     }
   }
 
-  // packed sign factor: pass 1 over the sign bits, G, pass 2 over their
-  // transpose, sums + stage epilogue
+  // packed sign factor: the stage vectors as digits, pass 1 over bitsT
+  // (rows m), G as digits, pass 2 over bits (rows n), sums + stage epilogue
   template <int KIND> void launch_sign(const Vecs<R> &v) {
     const int chk = v.check_state && KIND >= 2;
-    const long ldp1 = 32 * s_kw, ldp2 = 32 * s_nw;
+    const SignPass &p1 = s_pass[0], &p2 = s_pass[1];
     if (K > 0) {
-      sign_sum_kernel<R, R><<<dim3((unsigned)((s_kw + kBlock - 1) / kBlock), (unsigned)s_split1), kBlock, 0, stream>>>(
-          sbits, s_ldw, s_kw, N, s_per1, v.cs_in, v.cs_in + ldv, 1, s_part1, ldp1, st, chk);
-      sign_gram_kernel<R><<<(unsigned)((ldp1 + kBlock - 1) / kBlock), kBlock, 0, stream>>>(s_part1, s_split1, ldp1, w,
-                                                                                             K, s_G, st, chk);
-      sign_sum_kernel<R, double><<<dim3((unsigned)((s_nw + kBlock - 1) / kBlock), (unsigned)s_split2), kBlock, 0,
-                                   stream>>>(sbitsT, s_ldwT, s_nw, K, s_per2, s_G, s_G + 1, 2, s_part2, ldp2, st, chk);
+      const long n_pad = p1.nsb * kSignSB, k_pad = p2.nsb * kSignSB;
+      const long w1 = (p1.ldp / (16 * kSignNT)) * p1.nsplit, w2 = (p2.ldp / (16 * kSignNT)) * p2.nsplit;
+      constexpr int kWavesPerBlock = kBlock / kWave;
+      sign_limbs_kernel<R><<<(unsigned)((n_pad + kBlock - 1) / kBlock), kBlock, 0, stream>>>(
+          v.cs_in, v.cs_in + ldv, N, n_pad, p1.bfrag, st, chk);
+      sign_mfma_kernel<R, kSignNT><<<(unsigned)((w1 + kWavesPerBlock - 1) / kWavesPerBlock), kBlock, 0, stream>>>(
+          sbitsT, s_ldwT, K, N, (const u32x4 *)p1.bfrag, p1.nsb, p1.sb_per, p1.nsplit, p1.part, p1.ldp, st, chk);
+      sign_gram_kernel<R><<<(unsigned)((k_pad + kBlock - 1) / kBlock), kBlock, 0, stream>>>(
+          p1.part, p1.nsplit, p1.ldp, w, K, k_pad, p2.bfrag, st, chk);
+      sign_mfma_kernel<R, kSignNT><<<(unsigned)((w2 + kWavesPerBlock - 1) / kWavesPerBlock), kBlock, 0, stream>>>(
+          sbits, s_ldw, N, K, (const u32x4 *)p2.bfrag, p2.nsb, p2.sb_per, p2.nsplit, p2.part, p2.ldp, st, chk);
     }
     sign_stage_kernel<R, KIND><<<(unsigned)((N + kBlock - 1) / kBlock), kBlock, 0, stream>>>(
-        s_part2, K > 0 ? s_split2 : 0, ldp2, v);
+        p2.part, K > 0 ? p2.nsplit : 0, p2.ldp, v);
   }
 
   template <int KIND> void launch_factored(const Vecs<R> &v) {

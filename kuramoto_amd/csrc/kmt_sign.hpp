@@ -10,16 +10,23 @@
 //   bitsT [32 Kw][ldwT] row m, word q: n = 32 q + b in bit b (zero past N)
 // (1.25 GB each at N = K = 10^5, against 40 GB for X in FP32).
 //
-// Coupling (one RHS, four launches):
-//   sign_sum_kernel  pass 1 over bits:  partial sums of sum_n X_nm v_n for
-//                    every m (v = cos, sin), per range of n
-//   sign_gram_kernel G_m = w_m * (sum of the partials), zero past K
-//   sign_sum_kernel  pass 2 over bitsT: partial sums of sum_m X_nm G_m
-//   sign_stage_kernel C_n, S_n = sum of the partials, then the stage epilogue
-// Every sum is in double and runs in a fixed order (deterministic), and the
-// sums are those of the oracle's factored path (X_nm v_n and X_nm G_m are
-// +-v_n and +-G_m exactly), so only the order of the double additions
-// differs from it.
+// Coupling (one RHS, five launches), on the int8 matrix cores with exact
+// integer sums:
+//   sign_limbs_kernel  (cos, sin)_n as 56-bit fixed point (2^-54 units) in
+//                      7 signed base-256 digits each: the B operand of pass 1
+//   sign_mfma_kernel   pass 1 over bitsT: P_m = sum_n bit(m, n) * digit(n),
+//                      per digit and per range of n, plus T = sum_n digit(n)
+//   sign_gram_kernel   sum_n X_nm v_n = 2 P_m - T (X = 2 bit - 1) combined to
+//                      double, G_m = w_m * that, and G as digits: the B
+//                      operand of pass 2
+//   sign_mfma_kernel   pass 2 over bits: the same with rows n, k = m
+//   sign_stage_kernel  C_n, S_n = 2 P_n - T combined, then the stage epilogue
+// Each digit's sum over k is exact in int32 (|sum| <= 128 k < 2^31 for
+// k < 2^24), so every coupling sum is the exact sum of the fixed-point
+// inputs rounded once to double: of the oracle's factored path (kuramoto.cu:
+// 62-101 restated) it differs only by the inputs' rounding to 2^-54 (cos,
+// sin and |G_m| <= 1, as w_m = +-1/N) and by the oracle's own rounding of its
+// sequential double additions.  Deterministic.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -101,87 +108,187 @@ __device__ __forceinline__ uint32_t sign_stream_word(uint64_t W, uint32_t seed) 
   }
 }
 
-// Partial signed sums over a range of outer rows o (the rows of `bits`):
-//   part[split][32 q + b][0/1] = sum_o (bit b of bits[o][q] ? +1 : -1) * (vc_o, vs_o)
-// One thread per word q (32 inner indices, 64 double accumulators); the
-// outer vector is the same in every lane.  Formed as 2 P_b - T with
-// P_b = sum_o bit_b(o) v_o (an fma by the bit as 0.0 / 1.0: one bit-field
-// extract, one conversion and two fmas per entry) and T = sum_o v_o, shared
-// by the thread's 32 inner indices.  All in double; against the oracle's
-// sequential sum of +-v_o only the rounding of the double additions
-// differs (relative 1e-16 times the cancellation, ~sqrt of the row count).
-template <class R, class V>
-__global__ __launch_bounds__(kBlock) void sign_sum_kernel(const uint32_t *__restrict__ bits, long ldw, long nwords,
-                                                           long n_outer, long per_split, const V *__restrict__ vc,
-                                                           const V *__restrict__ vs, long vstride,
-                                                           double *__restrict__ part, long ldp, const State<R> *st,
-                                                           int check_state) {
+// ---------------------------------------------------------------------
+// The int8 passes.  An MFMA v_mfma_i32_16x16x64_i8 multiplies a 16 x 64 int8
+// A tile by a 64 x 16 int8 B tile into 16 x 16 int32: lane l holds A row l%16
+// and B column l%16 with 16 bytes each for its quarter g = l/16 of the 64 k,
+// and D rows 4g..4g+3 of column l%16.  A = the bits as bytes 0/1; B = 16
+// columns: the 7 digits of the cos (or G_c) value in 0..6, of sin (G_s) in
+// 7..13, zero in 14 and 15.
+//
+// k is walked in super-blocks of 512 (16 bit words per row).  Lane (row, g)
+// loads words 16 S + 4 g .. + 3 of its row as one 16-byte vector; for word u
+// (0..3) and half h (0, 1) the A operand is the four dwords
+// (word >> (4 h + j)) & 0x01010101, j = 0..3, so byte i of dword j is bit
+// 8 i + 4 h + j: k = 32 (16 S + 4 g + u) + 8 i + 4 h + j.  The B operand of
+// (S, u, h) is stored in exactly that order (sign_bfrag_offset), 1 KB per
+// MFMA, 8 KB per super-block, so the expansion of the bits costs two VALU
+// operations per 4 entries and no lane exchange.
+// ---------------------------------------------------------------------
+typedef int i32x4_t __attribute__((ext_vector_type(4)));
+constexpr int kSignSB = 512;                          // k per super-block
+constexpr long kSignBias = 0x80808080808080LL;        // 128 * sum_l 256^l, l < 7
+constexpr double kSignScale = 0x1p54;                 // fixed point: 2^-54 units
+// Largest N and K: a digit sum over k (2 P - T) is at most 3 * 128 k in
+// magnitude, 384 * 2^20 < 2^28.6 here, which keeps sign_combine exact.
+constexpr long kSignMaxDim = 1L << 20;
+
+// byte of column c (0..15) for index k in the B fragments
+__device__ __forceinline__ size_t sign_bfrag_offset(long k, int c) {
+  const long S = k >> 9;
+  const int w = (int)((k >> 5) & 15), g = w >> 2, u = w & 3;
+  const int b = (int)(k & 31), i = b >> 3, h = (b >> 2) & 1, j = b & 3;
+  return ((((size_t)S * 4 + u) * 2 + h) * 64 + (c + 16 * g)) * 16 + j * 4 + i;
+}
+
+// The 7 signed base-256 digits of round(v 2^54), |v| <= 1: e = d + bias is
+// non-negative and below 2^56, and digit l = byte l of e minus 128 (as a
+// byte: byte ^ 0x80), so that d = sum_l digit_l 256^l.
+__device__ __forceinline__ void sign_digits(double v, uint8_t *out) {
+  const long long e = __double2ll_rn(v * kSignScale) + kSignBias;
+#pragma unroll
+  for (int l = 0; l < 7; ++l) out[l] = (uint8_t)(((e >> (8 * l)) & 0xFF) ^ 0x80);
+}
+
+// sum_l s_l 256^l 2^-54 rounded once to double: |s_l| < 2^28.6, so the low
+// four digits (< 2^52.6) and the high three (< 2^44.6) are exact int64 and
+// exact doubles, and one fma rounds their sum.
+__device__ __forceinline__ double sign_combine(const long long (&s)[7]) {
+  const long long lo = s[0] + s[1] * 256LL + s[2] * 65536LL + s[3] * 16777216LL;
+  const long long hi = s[4] + s[5] * 256LL + s[6] * 65536LL;
+  return __builtin_fma((double)hi, 0x1p32, (double)lo) * 0x1p-54;
+}
+
+// B fragments of pass 1 from the stage vector pair: one thread per k = n
+// (zero columns 14, 15 and zero rows for k >= n_used up to the padded end).
+template <class R>
+__global__ __launch_bounds__(kBlock) void sign_limbs_kernel(const R *__restrict__ vc, const R *__restrict__ vs,
+                                                             long n_used, long n_pad, uint8_t *__restrict__ bfrag,
+                                                             const State<R> *st, int check_state) {
   if (check_state && (!st->active || st->commit_pending)) return;
-  const long q = (long)blockIdx.x * kBlock + threadIdx.x;
-  if (q >= nwords) return;
-  const long o0 = (long)blockIdx.y * per_split;
-  const long o1 = o0 + per_split < n_outer ? o0 + per_split : n_outer;
-  double pc[32], ps[32], tc = 0.0, ts = 0.0;
-#pragma unroll
-  for (int b = 0; b < 32; ++b) { pc[b] = 0.0; ps[b] = 0.0; }
-  auto row = [&](uint32_t wd, double c, double s) {
-    tc += c;
-    ts += s;
-#pragma unroll
-    for (int b = 0; b < 32; ++b) {
-      const double x = (double)((wd >> b) & 1u);
-      pc[b] = __builtin_fma(x, c, pc[b]);
-      ps[b] = __builtin_fma(x, s, ps[b]);
-    }
-  };
-  constexpr int U = 4;  // outer rows per batch of loads
-  long o = o0;
-  for (; o + U <= o1; o += U) {
-    uint32_t wd[U];
-    double c[U], s[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      wd[u] = __builtin_nontemporal_load(bits + (o + u) * ldw + q);
-      c[u] = (double)vc[(o + u) * vstride];
-      s[u] = (double)vs[(o + u) * vstride];
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) row(wd[u], c[u], s[u]);
+  const long k = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (k >= n_pad) return;
+  uint8_t d[16] = {};
+  if (k < n_used) {
+    sign_digits((double)vc[k], d);
+    sign_digits((double)vs[k], d + 7);
   }
-  for (; o < o1; ++o) row(bits[o * ldw + q], (double)vc[o * vstride], (double)vs[o * vstride]);
-  double *dst = part + ((size_t)blockIdx.y * ldp + 32 * q) * 2;
 #pragma unroll
-  for (int b = 0; b < 32; ++b) {
-    dst[2 * b] = 2.0 * pc[b] - tc;
-    dst[2 * b + 1] = 2.0 * ps[b] - ts;
+  for (int c = 0; c < 16; ++c) bfrag[sign_bfrag_offset(k, c)] = d[c];
+}
+
+__device__ __forceinline__ i32x4_t sign_expand(uint32_t w, int h) {
+  const uint32_t x = w >> (4 * h), m = 0x01010101u;
+  return i32x4_t{(int)(x & m), (int)((x >> 1) & m), (int)((x >> 2) & m), (int)((x >> 3) & m)};
+}
+
+// One pass: part[sp][r][c] = sum over the k of split sp of bit(r, k) * B[k][c]
+// for rows r < rows (bit rows of `bits`, ldw words each, a multiple of 16),
+// row `rows` = ones for k < kdim (T_c = sum_k B[k][c]), rows beyond zero.
+// One wave = NT 16-row tiles x one split of super-blocks; waves are
+// independent (no LDS, no barrier).
+template <class R, int NT>
+__global__ __launch_bounds__(kBlock) void sign_mfma_kernel(const uint32_t *__restrict__ bits, long ldw, long rows,
+                                                            long kdim, const u32x4 *__restrict__ bfrag, long nsb,
+                                                            long sb_per, int nsplit, int32_t *__restrict__ part,
+                                                            long ldp, const State<R> *st, int check_state) {
+  if (check_state && (!st->active || st->commit_pending)) return;
+  const long unit = ((long)blockIdx.x * kBlock + threadIdx.x) / kWave;
+  const int lane = threadIdx.x % kWave;
+  if (unit >= (ldp / (16 * NT)) * nsplit) return;  // whole waves
+  const long rg = unit / nsplit;
+  const int sp = (int)(unit % nsplit);
+  const long s0 = (long)sp * sb_per, s1 = s0 + sb_per < nsb ? s0 + sb_per : nsb;
+  const int g = lane >> 4, rl = lane & 15;
+  const uint32_t *rowp[NT];
+  int kind[NT];  // 0 = bit row, 1 = the ones row, 2 = zero
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const long r = rg * 16 * NT + 16 * t + rl;
+    kind[t] = r < rows ? 0 : (r == rows ? 1 : 2);
+    rowp[t] = bits + (r < rows ? r : 0) * ldw + 4 * g;
+  }
+  i32x4_t acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) acc[t] = i32x4_t{0, 0, 0, 0};
+  for (long S = s0; S < s1; ++S) {
+    u32x4 W[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      if (kind[t] == 0) {
+        W[t] = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(rowp[t] + 16 * S));
+      } else if (kind[t] == 1) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const long k0 = 32 * (16 * S + 4 * g + u);
+          W[t][u] = k0 + 32 <= kdim ? 0xFFFFFFFFu : (k0 < kdim ? (1u << (kdim - k0)) - 1u : 0u);
+        }
+      } else {
+        W[t] = u32x4{0u, 0u, 0u, 0u};
+      }
+    }
+    const u32x4 *bf = bfrag + (size_t)S * 8 * kWave + lane;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const i32x4_t b = __builtin_bit_cast(i32x4_t, bf[(u * 2 + h) * kWave]);
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+          acc[t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(sign_expand(W[t][u], h), b, acc[t], 0, 0, 0);
+      }
+  }
+#pragma unroll
+  for (int t = 0; t < NT; ++t)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const long r = rg * 16 * NT + 16 * t + 4 * g + v;
+      part[((size_t)sp * ldp + r) * 16 + rl] = acc[t][v];
+    }
+}
+
+// The 14 digit sums of row r (2 P - T over the splits), exact in int64.
+__device__ __forceinline__ void sign_row_sums(const int32_t *__restrict__ part, int nsplit, long ldp, long r,
+                                              long trow, long long (&sc)[7], long long (&ss)[7]) {
+#pragma unroll
+  for (int l = 0; l < 7; ++l) { sc[l] = 0; ss[l] = 0; }
+  for (int sp = 0; sp < nsplit; ++sp) {
+    const int32_t *pr = part + ((size_t)sp * ldp + r) * 16, *pt = part + ((size_t)sp * ldp + trow) * 16;
+#pragma unroll
+    for (int l = 0; l < 7; ++l) {
+      sc[l] += 2LL * pr[l] - pt[l];
+      ss[l] += 2LL * pr[7 + l] - pt[7 + l];
+    }
   }
 }
 
-// G_m = w_m * sum over the splits of the pass-1 partials (in split order),
-// interleaved (G_c, G_s); zero for m >= K.
+// G_m = w_m sum_n X_nm (cos, sin)_n from the pass-1 sums, written as the B
+// fragments of pass 2 (one thread per k = m, zero past K up to the padded end).
 template <class R>
-__global__ __launch_bounds__(kBlock) void sign_gram_kernel(const double *part, int nsplit, long ldp, const R *w,
-                                                            long K, double *G, const State<R> *st, int check_state) {
+__global__ __launch_bounds__(kBlock) void sign_gram_kernel(const int32_t *__restrict__ part, int nsplit, long ldp,
+                                                            const R *__restrict__ w, long K, long k_pad,
+                                                            uint8_t *__restrict__ bfrag, const State<R> *st,
+                                                            int check_state) {
   if (check_state && (!st->active || st->commit_pending)) return;
   const long m = (long)blockIdx.x * kBlock + threadIdx.x;
-  if (m >= ldp) return;
-  double c = 0.0, s = 0.0;
+  if (m >= k_pad) return;
+  uint8_t d[16] = {};
   if (m < K) {
-    for (int sp = 0; sp < nsplit; ++sp) {
-      c += part[((size_t)sp * ldp + m) * 2];
-      s += part[((size_t)sp * ldp + m) * 2 + 1];
-    }
-    c *= (double)w[m];
-    s *= (double)w[m];
+    long long sc[7], ss[7];
+    sign_row_sums(part, nsplit, ldp, m, K, sc, ss);
+    const double wm = (double)w[m];
+    sign_digits(sign_combine(sc) * wm, d);
+    sign_digits(sign_combine(ss) * wm, d + 7);
   }
-  G[2 * m] = c;
-  G[2 * m + 1] = s;
+#pragma unroll
+  for (int c = 0; c < 16; ++c) bfrag[sign_bfrag_offset(m, c)] = d[c];
 }
 
-// C_n, S_n = sum over the splits of the pass-2 partials, then the stage
-// epilogue of row n; stage 7 writes one error partial per workgroup.
+// C_n, S_n from the pass-2 sums, then the stage epilogue of row n; stage 7
+// writes one error partial per workgroup.
 template <class R, int KIND>
-__global__ __launch_bounds__(kBlock) void sign_stage_kernel(const double *part, int nsplit, long ldp, Vecs<R> v) {
+__global__ __launch_bounds__(kBlock) void sign_stage_kernel(const int32_t *__restrict__ part, int nsplit, long ldp,
+                                                             Vecs<R> v) {
   if constexpr (KIND >= 2) {
     if (v.check_state && (!v.st->active || v.st->commit_pending)) return;
   }
@@ -190,9 +297,11 @@ __global__ __launch_bounds__(kBlock) void sign_stage_kernel(const double *part, 
   double sq = 0.0;
   if (i < v.N) {
     double C = 0.0, S = 0.0;
-    for (int sp = 0; sp < nsplit; ++sp) {
-      C += part[((size_t)sp * ldp + i) * 2];
-      S += part[((size_t)sp * ldp + i) * 2 + 1];
+    if (nsplit > 0) {
+      long long sc[7], ss[7];
+      sign_row_sums(part, nsplit, ldp, i, v.N, sc, ss);
+      C = sign_combine(sc);
+      S = sign_combine(ss);
     }
     const R h = (KIND >= 2) ? v.st->h : (R)0;
     sq = row_epilogue<R, KIND>(v, i, (R)C, (R)S, h);
