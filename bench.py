@@ -397,15 +397,14 @@ def main():
     roof = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
             "frac": achieved / HBM_PEAK}
     if cfg["storage"] == "signs":
-        # two passes of signed sums over the N x K signs per RHS, each entry
-        # one signed addition per vector (cos, sin): 4 N K FP64 additions
-        # (the reference's m1 / m2 loops), issued as 4 VALU instructions per
-        # entry and pass (bit extract, conversion, two fmas)
-        flops = 4 * N * cfg["K"]
-        tf = flops / (r["stage_ms"] / 1e3) / 1e12
-        roof = {"bound": "valu", "achieved": tf, "peak": 78.6, "unit": "TFLOP/s", "frac": tf / 78.6,
-                "flops_per_launch": flops, "valu_instructions_per_launch": 8 * N * cfg["K"],
-                "valu_issue_frac": 8 * N * cfg["K"] / (r["stage_ms"] / 1e3) / (256 * 64 * 2.4e9)}
+        # two passes over the N x K signs per RHS on the int8 matrix cores:
+        # the bits (N K / 8 bytes per pass) streamed from HBM is the binding
+        # roof (the MFMA work, 16 digit columns x 2 ops per entry and pass,
+        # 64 N K int8 ops per RHS, takes 40% of the streaming time at the
+        # 5.0 POP/s dense int8 peak)
+        ops = 64 * N * cfg["K"]
+        roof.update({"int8_ops_per_launch": ops,
+                     "int8_pops": ops / (r["stage_ms"] / 1e3) / 1e15, "int8_peak_pops": 5.0})
     if cfg["storage"] == "rebuild":
         # (K products + K fmas to form M_ij, 2 fmas into C and S) per pair:
         # 3K + 4 flops in 2K + 3 VALU instructions per (i, j) (M_ij converted
@@ -444,8 +443,10 @@ def main():
         "roofline": {**roof,
                      "traffic": profiled_traffic(args.config, "lowrank_stage_kernel" if cfg["storage"] == "factored"
                                                  else "sym_tile_kernel" if cfg["storage"].startswith("sym")
+                                                 else "sign_mfma_kernel" if cfg["storage"] == "signs"
                                                  else "stage_kernel"),
-                     "kernel": ("sign_sum_kernel x 2 + sign_gram_kernel + sign_stage_kernel (one RHS)"
+                     "kernel": ("sign_mfma_kernel x 2 (int8 MFMA passes over the bits) + sign_limbs / sign_gram / "
+                                "sign_stage kernels (one RHS)"
                                 if cfg["storage"] == "signs" else
                                 "lowrank_stage_kernel (+ gram at stage 2; one RHS)" if cfg["storage"] == "factored"
                                 else "rebuild_stage_kernel (tiles rebuilt in registers)" if cfg["storage"] == "rebuild"

@@ -98,11 +98,13 @@ template <class R> struct Engine : EngineBase {
   uint32_t *sbits = nullptr, *sbitsT = nullptr;
   long s_kw = 0, s_ldw = 0, s_nw = 0, s_ldwT = 0;
   struct SignPass {
-    long ldp = 0, nsb = 0, sb_per = 1;
+    long ldp = 0, nsb = 0, npairs = 0, pairs_per = 1;  // nsb = 2 npairs super-blocks of 512
     int nsplit = 1;
     uint8_t *bfrag = nullptr;
     int32_t *part = nullptr;
+    uint32_t *ones = nullptr;  // the ones row: bits set for k < kdim, 16 nsb words
   } s_pass[2];
+  uint32_t *s_zero = nullptr;  // a zero row of max(16 nsb) words
   std::vector<R> X_host, w_host;
 
   // run
@@ -501,10 +503,11 @@ template <class R> struct Engine : EngineBase {
   }
 
   void drop_sign() {
-    void *b[] = {sbits, sbitsT, s_pass[0].bfrag, s_pass[0].part, s_pass[1].bfrag, s_pass[1].part};
+    void *b[] = {sbits, sbitsT, s_zero, s_pass[0].bfrag, s_pass[0].part, s_pass[0].ones,
+                 s_pass[1].bfrag, s_pass[1].part, s_pass[1].ones};
     for (void *p : b)
       if (p) (void)hipFree(p);
-    sbits = sbitsT = nullptr;
+    sbits = sbitsT = s_zero = nullptr;
     s_pass[0] = s_pass[1] = SignPass();
     sign = 0;
   }
@@ -512,22 +515,30 @@ template <class R> struct Engine : EngineBase {
   // 16-row tiles per wave of the int8 passes
   static constexpr int kSignNT = 4;
   // Shape of a pass with `rows` output rows (+ the ones row) over kdim: row
-  // groups of 16 kSignNT rows x splits of the 512-wide super-blocks, about
-  // 2048 waves (8 per CU) with at least 2 super-blocks per split.
+  // groups of 16 kSignNT rows x splits of the 1024-wide super-block pairs,
+  // about 8192 waves (4 rounds of the 2 waves per SIMD the kernel's
+  // registers allow, so the last round's tail is short) with at least 2
+  // pairs per split.
   static SignPass sign_pass_shape(long rows, long kdim) {
     SignPass p;
     const long rg = (rows + 1 + 16 * kSignNT - 1) / (16 * kSignNT);
     p.ldp = rg * 16 * kSignNT;
-    p.nsb = std::max<long>(1, (kdim + kSignSB - 1) / kSignSB);
-    long sp = std::max<long>(1, (2048 + rg - 1) / rg);
-    sp = std::min<long>(sp, std::max<long>(1, p.nsb / 2));
-    p.sb_per = (p.nsb + sp - 1) / sp;
-    p.nsplit = (int)((p.nsb + p.sb_per - 1) / p.sb_per);
+    p.npairs = std::max<long>(1, (kdim + 2 * kSignSB - 1) / (2 * kSignSB));
+    p.nsb = 2 * p.npairs;
+    long sp = std::max<long>(1, (8192 + rg - 1) / rg);
+    sp = std::min<long>(sp, std::max<long>(1, p.npairs / 2));
+    p.pairs_per = (p.npairs + sp - 1) / sp;
+    p.nsplit = (int)((p.npairs + p.pairs_per - 1) / p.pairs_per);
     return p;
   }
-  int alloc_sign_pass(SignPass &p) {
+  int alloc_sign_pass(SignPass &p, long kdim) {
     HIP_TRY(hipMalloc((void **)&p.bfrag, (size_t)p.nsb * 8 * kWave * 16));
     HIP_TRY(hipMalloc((void **)&p.part, sizeof(int32_t) * 16 * (size_t)p.nsplit * p.ldp));
+    std::vector<uint32_t> ones((size_t)16 * p.nsb, 0u);
+    for (long q = 0; q < (long)ones.size(); ++q)
+      ones[q] = 32 * (q + 1) <= kdim ? 0xFFFFFFFFu : (32 * q < kdim ? (1u << (kdim - 32 * q)) - 1u : 0u);
+    HIP_TRY(hipMalloc((void **)&p.ones, sizeof(uint32_t) * ones.size()));
+    HIP_TRY(hipMemcpy(p.ones, ones.data(), sizeof(uint32_t) * ones.size(), hipMemcpyHostToDevice));
     return KMT_OK;
   }
 
@@ -553,7 +564,7 @@ template <class R> struct Engine : EngineBase {
     HIP_TRY(hipMalloc((void **)&w, sizeof(R) * std::max(K, 1)));
     if (K > 0) HIP_TRY(hipMemcpyAsync(w, w_host.data(), sizeof(R) * K, hipMemcpyHostToDevice, stream));
     s_kw = (K + 31) / 32;
-    s_ldw = round_up(std::max<long>(s_kw, 1), 16);  // whole super-blocks of the pass-2 rows
+    s_ldw = round_up(std::max<long>(s_kw, 1), 32);  // whole super-block pairs of the pass-2 rows
     s_nw = (N + 31) / 32;
     s_ldwT = round_up(s_nw, 64);
     hipError_t e = hipMalloc((void **)&sbits, sizeof(uint32_t) * (size_t)N * s_ldw);
@@ -564,10 +575,13 @@ template <class R> struct Engine : EngineBase {
     }
     s_pass[0] = sign_pass_shape(K, N);
     s_pass[1] = sign_pass_shape(N, K);
-    if (alloc_sign_pass(s_pass[0]) != KMT_OK || alloc_sign_pass(s_pass[1]) != KMT_OK) {
+    const long zwords = 16 * std::max(s_pass[0].nsb, s_pass[1].nsb);
+    if (alloc_sign_pass(s_pass[0], N) != KMT_OK || alloc_sign_pass(s_pass[1], K) != KMT_OK ||
+        hipMalloc((void **)&s_zero, sizeof(uint32_t) * zwords) != hipSuccess) {
       drop_sign();
       return fail(KMT_ERR_MEMORY, "cannot allocate the sign passes' buffers (N=%ld, K=%d)", N, K);
     }
+    HIP_TRY(hipMemsetAsync(s_zero, 0, sizeof(uint32_t) * zwords, stream));
     // words of bitsT past N are never written by the transpose: zero them
     HIP_TRY(hipMemsetAsync(sbitsT, 0, sizeof(uint32_t) * (size_t)std::max<long>(32 * s_kw, 1) * s_ldwT, stream));
     const size_t words = (size_t)N * s_ldw;
@@ -858,11 +872,13 @@ template <class R> struct Engine : EngineBase {
       sign_limbs_kernel<R><<<(unsigned)((n_pad + kBlock - 1) / kBlock), kBlock, 0, stream>>>(
           v.cs_in, v.cs_in + ldv, N, n_pad, p1.bfrag, st, chk);
       sign_mfma_kernel<R, kSignNT><<<(unsigned)((w1 + kWavesPerBlock - 1) / kWavesPerBlock), kBlock, 0, stream>>>(
-          sbitsT, s_ldwT, K, N, (const u32x4 *)p1.bfrag, p1.nsb, p1.sb_per, p1.nsplit, p1.part, p1.ldp, st, chk);
+          sbitsT, s_ldwT, K, p1.ones, s_zero, (const u32x4 *)p1.bfrag, p1.npairs, p1.pairs_per, p1.nsplit,
+          p1.part, p1.ldp, st, chk);
       sign_gram_kernel<R><<<(unsigned)((k_pad + kBlock - 1) / kBlock), kBlock, 0, stream>>>(
           p1.part, p1.nsplit, p1.ldp, w, K, k_pad, p2.bfrag, st, chk);
       sign_mfma_kernel<R, kSignNT><<<(unsigned)((w2 + kWavesPerBlock - 1) / kWavesPerBlock), kBlock, 0, stream>>>(
-          sbits, s_ldw, N, K, (const u32x4 *)p2.bfrag, p2.nsb, p2.sb_per, p2.nsplit, p2.part, p2.ldp, st, chk);
+          sbits, s_ldw, N, p2.ones, s_zero, (const u32x4 *)p2.bfrag, p2.npairs, p2.pairs_per, p2.nsplit, p2.part,
+          p2.ldp, st, chk);
     }
     sign_stage_kernel<R, KIND><<<(unsigned)((N + kBlock - 1) / kBlock), kBlock, 0, stream>>>(
         p2.part, K > 0 ? p2.nsplit : 0, p2.ldp, v);

@@ -116,14 +116,16 @@ __device__ __forceinline__ uint32_t sign_stream_word(uint64_t W, uint32_t seed) 
 // columns: the 7 digits of the cos (or G_c) value in 0..6, of sin (G_s) in
 // 7..13, zero in 14 and 15.
 //
-// k is walked in super-blocks of 512 (16 bit words per row).  Lane (row, g)
-// loads words 16 S + 4 g .. + 3 of its row as one 16-byte vector; for word u
-// (0..3) and half h (0, 1) the A operand is the four dwords
-// (word >> (4 h + j)) & 0x01010101, j = 0..3, so byte i of dword j is bit
-// 8 i + 4 h + j: k = 32 (16 S + 4 g + u) + 8 i + 4 h + j.  The B operand of
-// (S, u, h) is stored in exactly that order (sign_bfrag_offset), 1 KB per
-// MFMA, 8 KB per super-block, so the expansion of the bits costs two VALU
-// operations per 4 entries and no lane exchange.
+// k is walked in pairs of 512-wide super-blocks (32 bit words = one 128-byte
+// line per row).  Lane (row, g) loads words 32 P + 8 g .. + 7 of its row as
+// two 16-byte vectors, the 4 lanes of a row one whole line; super-block
+// S = 2 P + s takes words 32 P + 8 g + 4 s + u, u = 0..3.  For word u and
+// half h (0, 1) the A operand is the four dwords (word >> (4 h + j)) &
+// 0x01010101, j = 0..3, so byte i of dword j is bit 8 i + 4 h + j:
+// k = 32 (32 P + 8 g + 4 s + u) + 8 i + 4 h + j.  The B operand of (S, u, h)
+// is stored in exactly that order (sign_bfrag_offset), 1 KB per MFMA, 8 KB
+// per super-block, so the expansion of the bits costs two VALU operations
+// per 4 entries and no lane exchange.
 // ---------------------------------------------------------------------
 typedef int i32x4_t __attribute__((ext_vector_type(4)));
 constexpr int kSignSB = 512;                          // k per super-block
@@ -135,8 +137,9 @@ constexpr long kSignMaxDim = 1L << 20;
 
 // byte of column c (0..15) for index k in the B fragments
 __device__ __forceinline__ size_t sign_bfrag_offset(long k, int c) {
-  const long S = k >> 9;
-  const int w = (int)((k >> 5) & 15), g = w >> 2, u = w & 3;
+  const long q = k >> 5;
+  const long S = 2 * (q >> 5) + ((q >> 2) & 1);
+  const int g = (int)((q >> 3) & 3), u = (int)(q & 3);
   const int b = (int)(k & 31), i = b >> 3, h = (b >> 2) & 1, j = b & 3;
   return ((((size_t)S * 4 + u) * 2 + h) * 64 + (c + 16 * g)) * 16 + j * 4 + i;
 }
@@ -183,14 +186,16 @@ __device__ __forceinline__ i32x4_t sign_expand(uint32_t w, int h) {
 }
 
 // One pass: part[sp][r][c] = sum over the k of split sp of bit(r, k) * B[k][c]
-// for rows r < rows (bit rows of `bits`, ldw words each, a multiple of 16),
-// row `rows` = ones for k < kdim (T_c = sum_k B[k][c]), rows beyond zero.
-// One wave = NT 16-row tiles x one split of super-blocks; waves are
-// independent (no LDS, no barrier).
+// for rows r < rows (bit rows of `bits`, ldw words each, a multiple of 32),
+// row `rows` = ones_row (ones for k < kdim: T_c = sum_k B[k][c]), rows
+// beyond: zero_row (both ldw words).  One wave = NT 16-row tiles x one
+// split of super-block pairs; waves are independent (no LDS, no barrier).
 template <class R, int NT>
 __global__ __launch_bounds__(kBlock) void sign_mfma_kernel(const uint32_t *__restrict__ bits, long ldw, long rows,
-                                                            long kdim, const u32x4 *__restrict__ bfrag, long nsb,
-                                                            long sb_per, int nsplit, int32_t *__restrict__ part,
+                                                            const uint32_t *__restrict__ ones_row,
+                                                            const uint32_t *__restrict__ zero_row,
+                                                            const u32x4 *__restrict__ bfrag, long npairs,
+                                                            long pairs_per, int nsplit, int32_t *__restrict__ part,
                                                             long ldp, const State<R> *st, int check_state) {
   if (check_state && (!st->active || st->commit_pending)) return;
   const long unit = ((long)blockIdx.x * kBlock + threadIdx.x) / kWave;
@@ -198,45 +203,82 @@ __global__ __launch_bounds__(kBlock) void sign_mfma_kernel(const uint32_t *__res
   if (unit >= (ldp / (16 * NT)) * nsplit) return;  // whole waves
   const long rg = unit / nsplit;
   const int sp = (int)(unit % nsplit);
-  const long s0 = (long)sp * sb_per, s1 = s0 + sb_per < nsb ? s0 + sb_per : nsb;
+  const long p0 = (long)sp * pairs_per, p1 = p0 + pairs_per < npairs ? p0 + pairs_per : npairs;
   const int g = lane >> 4, rl = lane & 15;
+  // Every lane loads (no divergent branch around the loads, which would make
+  // the compiler wait for every load in flight): the ones and zero rows are
+  // real rows in memory.
   const uint32_t *rowp[NT];
-  int kind[NT];  // 0 = bit row, 1 = the ones row, 2 = zero
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
     const long r = rg * 16 * NT + 16 * t + rl;
-    kind[t] = r < rows ? 0 : (r == rows ? 1 : 2);
-    rowp[t] = bits + (r < rows ? r : 0) * ldw + 4 * g;
+    rowp[t] = (r < rows ? bits + r * ldw : (r == rows ? ones_row : zero_row)) + 8 * g;
   }
   i32x4_t acc[NT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) acc[t] = i32x4_t{0, 0, 0, 0};
-  for (long S = s0; S < s1; ++S) {
-    u32x4 W[NT];
+  const u32x4 *bf = bfrag + lane;
+  // pair P: the rows' bit words (nothing here consumes a loaded value, so
+  // the loads stay in flight); super-block S: its 8 B fragments
+  auto load_w = [&](long P, u32x4 (&W)[NT][2]) {
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-      if (kind[t] == 0) {
-        W[t] = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(rowp[t] + 16 * S));
-      } else if (kind[t] == 1) {
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const long k0 = 32 * (16 * S + 4 * g + u);
-          W[t][u] = k0 + 32 <= kdim ? 0xFFFFFFFFu : (k0 < kdim ? (1u << (kdim - k0)) - 1u : 0u);
-        }
-      } else {
-        W[t] = u32x4{0u, 0u, 0u, 0u};
-      }
+      W[t][0] = *reinterpret_cast<const u32x4 *>(rowp[t] + 32 * P);
+      W[t][1] = *reinterpret_cast<const u32x4 *>(rowp[t] + 32 * P + 4);
     }
-    const u32x4 *bf = bfrag + (size_t)S * 8 * kWave + lane;
+  };
+  auto load_b = [&](long S, u32x4 (&B)[8]) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) B[q] = bf[((size_t)S * 8 + q) * kWave];
+  };
+  // the MFMAs of super-block 2 P + s, the words expanded to bytes per (word, half)
+  auto multiply = [&](const u32x4 (&W)[NT][2], int s, const u32x4 (&B)[8]) {
 #pragma unroll
     for (int u = 0; u < 4; ++u)
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
-        const i32x4_t b = __builtin_bit_cast(i32x4_t, bf[(u * 2 + h) * kWave]);
+        const i32x4_t b = __builtin_bit_cast(i32x4_t, B[u * 2 + h]);
 #pragma unroll
         for (int t = 0; t < NT; ++t)
-          acc[t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(sign_expand(W[t][u], h), b, acc[t], 0, 0, 0);
+          acc[t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(sign_expand(W[t][s][u], h), b, acc[t], 0, 0, 0);
       }
+  };
+  // Bit words one pair ahead, B fragments one super-block ahead, in two
+  // register sets each; loads past the split's end re-read its last pair /
+  // super-block (harmless, and no branch).  The scheduling barriers keep
+  // every load ahead of the MFMAs that do not need it.
+  const long plast = p1 - 1, slast = 2 * p1 - 1;
+  auto cl = [](long x, long hi) { return x < hi ? x : hi; };
+  u32x4 Wa[NT][2], Wb[NT][2], B0[8], B1[8];
+  long P = p0;
+  if (p0 < p1) {
+    load_w(p0, Wa);
+    load_b(2 * p0, B0);
+  }
+  for (; P + 2 <= p1; P += 2) {
+    load_w(P + 1, Wb);
+    load_b(2 * P + 1, B1);
+    __builtin_amdgcn_sched_barrier(0);
+    multiply(Wa, 0, B0);
+    __builtin_amdgcn_sched_barrier(0);
+    load_b(2 * P + 2, B0);
+    __builtin_amdgcn_sched_barrier(0);
+    multiply(Wa, 1, B1);
+    __builtin_amdgcn_sched_barrier(0);
+    load_w(cl(P + 2, plast), Wa);
+    load_b(2 * P + 3, B1);
+    __builtin_amdgcn_sched_barrier(0);
+    multiply(Wb, 0, B0);
+    __builtin_amdgcn_sched_barrier(0);
+    load_b(cl(2 * P + 4, slast), B0);
+    __builtin_amdgcn_sched_barrier(0);
+    multiply(Wb, 1, B1);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  if (P < p1) {
+    load_b(2 * P + 1, B1);
+    multiply(Wa, 0, B0);
+    multiply(Wa, 1, B1);
   }
 #pragma unroll
   for (int t = 0; t < NT; ++t)
