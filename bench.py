@@ -47,7 +47,7 @@ def profiled_traffic(config, kernel="stage_kernel"):
         return None
     for line in open(path):
         if line.startswith("traffic_bytes_per_launch:"):
-            return int(line.split(":")[1])
+            return int(line.split(":")[1].split()[0])
     return None
 
 
