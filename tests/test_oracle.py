@@ -44,6 +44,28 @@ def test_frequency_kat_discriminates_typing(orc):
     assert not np.array_equal(om32, om64.astype(np.float32))
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_initial_phase_draw_range_and_moments(orc, dtype):
+    """kuramoto.cu:499 (kuramoto_64.cu the same): theta_j = I/2 (2 u - 1),
+    u = rand()/RAND_MAX, i.e. uniform on [-I/2, I/2]: every draw inside it,
+    the extremes reached, mean 0 and variance I^2/12 (the I/2 scale: a
+    dropped halving would double the range and quadruple the variance),
+    and I = 0 gives zeros."""
+    N, I = 400000, 6.283185307
+    th, _ = orc.draws(N, 11, False, 1.0, I, dtype=dtype)
+    th = th.astype(np.float64)
+    half = np.float64(dtype(I) / 2) if dtype == np.float32 else I / 2
+    assert th.min() >= -half * (1 + 1e-7) and th.max() <= half * (1 + 1e-7)
+    assert th.min() < -0.999 * half and th.max() > 0.999 * half
+    sd = I / np.sqrt(12.0)
+    assert abs(th.mean()) < 5 * sd / np.sqrt(N)
+    assert abs(th.var() / (I * I / 12) - 1) < 0.01
+    # uniform: a fourth moment of 9/5 sigma^4 (a normal would give 3)
+    assert abs(np.mean(th ** 4) / sd ** 4 - 1.8) < 0.02
+    th0, _ = orc.draws(1000, 11, False, 1.0, 0.0, dtype=dtype)
+    assert not np.any(th0)
+
+
 def test_gaussian_factor_moments(orc):
     X = orc.gaussian_factor(200000, 5, 3)
     assert abs(X.mean()) < 0.005 and abs(X.std() - 1) < 0.005
