@@ -132,10 +132,12 @@ int kmt_set_factor(kmt_ctx *ctx, const void *X, const void *w, int K);
  * and store none), kept packed at one bit per entry plus its transpose
  * (N K / 4 bytes in all, 2.5 GB at N = K = 10^5) instead of an N x K real
  * array from the host.  Afterwards kmt_use_factored(ctx, KMT_LOWRANK_CONTRACT)
- * couples through it (two passes of signed sums in double per RHS, any K),
- * and kmt_build_adjacency builds the dense matrix from it (native, BF16 or
- * FP16 storage) with the same fma chain as from a real factor.  Replaces a
- * previous kmt_set_factor; no register-rebuild form. */
+ * couples through it (two passes per RHS on the int8 matrix cores: the
+ * vectors as 56-bit fixed point, exact integer sums, one rounding to
+ * double), and kmt_build_adjacency builds the dense matrix from it (native,
+ * BF16 or FP16 storage) with the same fma chain as from a real factor.
+ * N and K up to 2^20 (KMT_ERR_ARG beyond).  Replaces a previous
+ * kmt_set_factor; no register-rebuild form. */
 int kmt_set_factor_sign(kmt_ctx *ctx, int K, uint32_t seed);
 /* Build the dense adjacency on the device from the factor (getadj,
  * kuramoto.cu:145-161): M_ij = sum_m fma(w_m, X_im X_jm, .) in the working
