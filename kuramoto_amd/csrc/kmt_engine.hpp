@@ -517,15 +517,16 @@ template <class R> struct Engine : EngineBase {
 
   // 16-row tiles per wave of the int8 passes
   static constexpr int kSignNT = 4;
-  // Shape of a pass with `rows` output rows (+ the ones row) over kdim: row
-  // groups of 16 kSignNT rows x splits of the 1024-wide super-block pairs,
-  // about 8192 waves (4 rounds of the 2 waves per SIMD the kernel's
-  // registers allow, so the last round's tail is short) with at least 2
-  // pairs per split.
+  // Shape of a pass with `rows` output rows (+ the ones row) over kdim:
+  // workgroups of 4 row groups of 16 kSignNT rows x splits of the 1024-wide
+  // super-block pairs, about 8192 waves (4 rounds of the 2 waves per SIMD
+  // the kernel's registers allow, so the last round's tail is short) with
+  // at least 2 pairs per split.
+  static constexpr long kSignRowsPerBlock = 16 * kSignNT * (kBlock / kWave);
   static SignPass sign_pass_shape(long rows, long kdim) {
     SignPass p;
-    const long rg = (rows + 1 + 16 * kSignNT - 1) / (16 * kSignNT);
-    p.ldp = rg * 16 * kSignNT;
+    p.ldp = (rows + 1 + kSignRowsPerBlock - 1) / kSignRowsPerBlock * kSignRowsPerBlock;
+    const long rg = p.ldp / (16 * kSignNT);
     p.npairs = std::max<long>(1, (kdim + 2 * kSignSB - 1) / (2 * kSignSB));
     p.nsb = 2 * p.npairs;
     long sp = std::max<long>(1, (8192 + rg - 1) / rg);
@@ -870,16 +871,15 @@ template <class R> struct Engine : EngineBase {
     const SignPass &p1 = s_pass[0], &p2 = s_pass[1];
     if (K > 0) {
       const long n_pad = p1.nsb * kSignSB, k_pad = p2.nsb * kSignSB;
-      const long w1 = (p1.ldp / (16 * kSignNT)) * p1.nsplit, w2 = (p2.ldp / (16 * kSignNT)) * p2.nsplit;
-      constexpr int kWavesPerBlock = kBlock / kWave;
+      const long b1 = (p1.ldp / kSignRowsPerBlock) * p1.nsplit, b2 = (p2.ldp / kSignRowsPerBlock) * p2.nsplit;
       sign_limbs_kernel<R><<<(unsigned)((n_pad + kBlock - 1) / kBlock), kBlock, 0, stream>>>(
           v.cs_in, v.cs_in + ldv, N, n_pad, p1.bfrag, st, chk);
-      sign_mfma_kernel<R, kSignNT><<<(unsigned)((w1 + kWavesPerBlock - 1) / kWavesPerBlock), kBlock, 0, stream>>>(
+      sign_mfma_kernel<R, kSignNT><<<(unsigned)b1, kBlock, 0, stream>>>(
           sbitsT, s_ldwT, K, p1.ones, s_zero, (const u32x4 *)p1.bfrag, p1.npairs, p1.pairs_per, p1.nsplit,
           p1.part, p1.ldp, st, chk);
       sign_gram_kernel<R><<<(unsigned)((k_pad + kBlock - 1) / kBlock), kBlock, 0, stream>>>(
           p1.part, p1.nsplit, p1.ldp, w, K, k_pad, p2.bfrag, st, chk);
-      sign_mfma_kernel<R, kSignNT><<<(unsigned)((w2 + kWavesPerBlock - 1) / kWavesPerBlock), kBlock, 0, stream>>>(
+      sign_mfma_kernel<R, kSignNT><<<(unsigned)b2, kBlock, 0, stream>>>(
           sbits, s_ldw, N, p2.ones, s_zero, (const u32x4 *)p2.bfrag, p2.npairs, p2.pairs_per, p2.nsplit, p2.part,
           p2.ldp, st, chk);
     }

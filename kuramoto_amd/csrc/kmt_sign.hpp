@@ -31,6 +31,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "kmt_kernels.hpp"
 
 namespace kmt {
@@ -128,6 +130,14 @@ __device__ __forceinline__ uint32_t sign_stream_word(uint64_t W, uint32_t seed) 
 // per 4 entries and no lane exchange.
 // ---------------------------------------------------------------------
 typedef int i32x4_t __attribute__((ext_vector_type(4)));
+constexpr int kSignPF = 3;  // pairs of bit words in flight per wave
+
+template <int I, int U, class F> __device__ __forceinline__ void sign_unroll(F &&f) {
+  if constexpr (I < U) {
+    f(std::integral_constant<int, I>{});
+    sign_unroll<I + 1, U>(f);
+  }
+}
 constexpr int kSignSB = 512;                          // k per super-block
 constexpr long kSignBias = 0x80808080808080LL;        // 128 * sum_l 256^l, l < 7
 constexpr double kSignScale = 0x1p54;                 // fixed point: 2^-54 units
@@ -188,8 +198,14 @@ __device__ __forceinline__ i32x4_t sign_expand(uint32_t w, int h) {
 // One pass: part[sp][r][c] = sum over the k of split sp of bit(r, k) * B[k][c]
 // for rows r < rows (bit rows of `bits`, ldw words each, a multiple of 32),
 // row `rows` = ones_row (ones for k < kdim: T_c = sum_k B[k][c]), rows
-// beyond: zero_row (both ldw words).  One wave = NT 16-row tiles x one
-// split of super-block pairs; waves are independent (no LDS, no barrier).
+// beyond: zero_row (both ldw words).  A workgroup = 4 waves on the same
+// split of super-block pairs; wave w owns row group 4 (group of blocks) + w,
+// NT 16-row tiles.  The B fragments of a pair (16 KB) are loaded once per
+// workgroup, 64 bytes per thread, and shared through LDS (double-buffered,
+// one barrier per pair); every thread's loads for pair P + kSignPF (its
+// share of the B fragments before the MFMAs of pair P, its rows' bit words
+// after them) are issued in the order they are used, so kSignPF - 1 pairs
+// stay in flight behind the one in use.
 template <class R, int NT>
 __global__ __launch_bounds__(kBlock) void sign_mfma_kernel(const uint32_t *__restrict__ bits, long ldw, long rows,
                                                             const uint32_t *__restrict__ ones_row,
@@ -197,12 +213,14 @@ __global__ __launch_bounds__(kBlock) void sign_mfma_kernel(const uint32_t *__res
                                                             const u32x4 *__restrict__ bfrag, long npairs,
                                                             long pairs_per, int nsplit, int32_t *__restrict__ part,
                                                             long ldp, const State<R> *st, int check_state) {
+  constexpr int NW = kBlock / kWave;
+  constexpr int BPT = 2 * 8 * kWave / kBlock;  // 16-byte B pieces per thread and pair
+  __shared__ u32x4 bsh[2][2 * 8 * kWave];
   if (check_state && (!st->active || st->commit_pending)) return;
-  const long unit = ((long)blockIdx.x * kBlock + threadIdx.x) / kWave;
-  const int lane = threadIdx.x % kWave;
-  if (unit >= (ldp / (16 * NT)) * nsplit) return;  // whole waves
-  const long rg = unit / nsplit;
-  const int sp = (int)(unit % nsplit);
+  const int tid = threadIdx.x, lane = tid % kWave, wave = tid / kWave;
+  const long rgb = blockIdx.x / nsplit;  // the workgroup's 4 row groups
+  const int sp = (int)(blockIdx.x % nsplit);
+  const long rg = rgb * NW + wave;
   const long p0 = (long)sp * pairs_per, p1 = p0 + pairs_per < npairs ? p0 + pairs_per : npairs;
   const int g = lane >> 4, rl = lane & 15;
   // Every lane loads (no divergent branch around the loads, which would make
@@ -217,9 +235,9 @@ __global__ __launch_bounds__(kBlock) void sign_mfma_kernel(const uint32_t *__res
   i32x4_t acc[NT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) acc[t] = i32x4_t{0, 0, 0, 0};
-  const u32x4 *bf = bfrag + lane;
-  // pair P: the rows' bit words (nothing here consumes a loaded value, so
-  // the loads stay in flight); super-block S: its 8 B fragments
+  // pair P: the rows' bit words and this thread's pieces of the pair's B
+  // fragments (piece f = tid + kBlock j of the pair's 1024, in the pair's
+  // order: super-block s, (u, h), lane).  Nothing here consumes a value.
   auto load_w = [&](long P, u32x4 (&W)[NT][2]) {
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
@@ -227,59 +245,55 @@ __global__ __launch_bounds__(kBlock) void sign_mfma_kernel(const uint32_t *__res
       W[t][1] = *reinterpret_cast<const u32x4 *>(rowp[t] + 32 * P + 4);
     }
   };
-  auto load_b = [&](long S, u32x4 (&B)[8]) {
+  auto load_b = [&](long P, u32x4 (&Bst)[BPT]) {
 #pragma unroll
-    for (int q = 0; q < 8; ++q) B[q] = bf[((size_t)S * 8 + q) * kWave];
+    for (int j = 0; j < BPT; ++j) Bst[j] = bfrag[(size_t)P * 2 * 8 * kWave + tid + kBlock * j];
   };
-  // the MFMAs of super-block 2 P + s, the words expanded to bytes per (word, half)
-  auto multiply = [&](const u32x4 (&W)[NT][2], int s, const u32x4 (&B)[8]) {
+  // the MFMAs of super-block 2 P + s, B fragments from LDS buffer buf
+  auto multiply = [&](const u32x4 (&W)[NT][2], int s, int buf) {
 #pragma unroll
     for (int u = 0; u < 4; ++u)
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
-        const i32x4_t b = __builtin_bit_cast(i32x4_t, B[u * 2 + h]);
+        const i32x4_t b = __builtin_bit_cast(i32x4_t, bsh[buf][(s * 8 + u * 2 + h) * kWave + lane]);
 #pragma unroll
         for (int t = 0; t < NT; ++t)
           acc[t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(sign_expand(W[t][s][u], h), b, acc[t], 0, 0, 0);
       }
   };
-  // Bit words one pair ahead, B fragments one super-block ahead, in two
-  // register sets each; loads past the split's end re-read its last pair /
-  // super-block (harmless, and no branch).  The scheduling barriers keep
-  // every load ahead of the MFMAs that do not need it.
-  const long plast = p1 - 1, slast = 2 * p1 - 1;
+  // Ring of kSignPF register sets, slot d holding pair P + d.  Loads past
+  // the split's end re-read its last pair (harmless, and no branch).
+  const long plast = p1 - 1;
   auto cl = [](long x, long hi) { return x < hi ? x : hi; };
-  u32x4 Wa[NT][2], Wb[NT][2], B0[8], B1[8];
+  u32x4 W[kSignPF][NT][2], Bst[kSignPF][BPT];
+  if (p0 < p1)
+    sign_unroll<0, kSignPF>([&](auto i) {
+      constexpr int d = decltype(i)::value;
+      load_b(cl(p0 + d, plast), Bst[d]);
+      load_w(cl(p0 + d, plast), W[d]);
+    });
+  // pair P + d from slot d: its B fragments to LDS and a barrier, the slot's
+  // B pieces refilled with pair P + d + kSignPF, the pair's MFMAs, then the
+  // slot's bit words refilled (after the MFMAs that read them)
+  auto pair = [&](auto i, long P, bool refill) {
+    constexpr int d = decltype(i)::value;
+    const int buf = (int)((P + d - p0) & 1);
+#pragma unroll
+    for (int j = 0; j < BPT; ++j) bsh[buf][tid + kBlock * j] = Bst[d][j];
+    __syncthreads();
+    if (refill) load_b(cl(P + d + kSignPF, plast), Bst[d]);
+    __builtin_amdgcn_sched_barrier(0);
+    multiply(W[d], 0, buf);
+    multiply(W[d], 1, buf);
+    __builtin_amdgcn_sched_barrier(0);
+    if (refill) load_w(cl(P + d + kSignPF, plast), W[d]);
+    __builtin_amdgcn_sched_barrier(0);
+  };
   long P = p0;
-  if (p0 < p1) {
-    load_w(p0, Wa);
-    load_b(2 * p0, B0);
-  }
-  for (; P + 2 <= p1; P += 2) {
-    load_w(P + 1, Wb);
-    load_b(2 * P + 1, B1);
-    __builtin_amdgcn_sched_barrier(0);
-    multiply(Wa, 0, B0);
-    __builtin_amdgcn_sched_barrier(0);
-    load_b(2 * P + 2, B0);
-    __builtin_amdgcn_sched_barrier(0);
-    multiply(Wa, 1, B1);
-    __builtin_amdgcn_sched_barrier(0);
-    load_w(cl(P + 2, plast), Wa);
-    load_b(2 * P + 3, B1);
-    __builtin_amdgcn_sched_barrier(0);
-    multiply(Wb, 0, B0);
-    __builtin_amdgcn_sched_barrier(0);
-    load_b(cl(2 * P + 4, slast), B0);
-    __builtin_amdgcn_sched_barrier(0);
-    multiply(Wb, 1, B1);
-    __builtin_amdgcn_sched_barrier(0);
-  }
-  if (P < p1) {
-    load_b(2 * P + 1, B1);
-    multiply(Wa, 0, B0);
-    multiply(Wa, 1, B1);
-  }
+  for (; P + kSignPF <= p1; P += kSignPF) sign_unroll<0, kSignPF>([&](auto i) { pair(i, P, true); });
+  sign_unroll<0, kSignPF - 1>([&](auto i) {
+    if (P + decltype(i)::value < p1) pair(i, P, false);
+  });
 #pragma unroll
   for (int t = 0; t < NT; ++t)
 #pragma unroll
