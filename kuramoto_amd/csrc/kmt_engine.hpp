@@ -519,9 +519,9 @@ template <class R> struct Engine : EngineBase {
   static constexpr int kSignNT = 4;
   // Shape of a pass with `rows` output rows (+ the ones row) over kdim:
   // workgroups of 4 row groups of 16 kSignNT rows x splits of the 1024-wide
-  // super-block pairs, about 8192 waves (4 rounds of the 2 waves per SIMD
-  // the kernel's registers allow, so the last round's tail is short) with
-  // at least 2 pairs per split.
+  // super-block pairs, about 8192 waves (2.7 rounds of the 3 waves per SIMD
+  // the kernel's registers allow, so the last round's tail is short; 12288
+  // measured the same) with at least 2 pairs per split.
   static constexpr long kSignRowsPerBlock = 16 * kSignNT * (kBlock / kWave);
   static SignPass sign_pass_shape(long rows, long kdim) {
     SignPass p;

@@ -130,7 +130,10 @@ __device__ __forceinline__ uint32_t sign_stream_word(uint64_t W, uint32_t seed) 
 // per 4 entries and no lane exchange.
 // ---------------------------------------------------------------------
 typedef int i32x4_t __attribute__((ext_vector_type(4)));
-constexpr int kSignPF = 3;  // pairs of bit words in flight per wave
+// Register sets of the pass's ring: one pair in use, kSignPF - 1 in flight.
+// 2 sets keep 3 waves per SIMD (132 VGPRs); 3 sets (182 VGPRs, 2 waves) ran
+// 5% slower at N = K = 10^5, 4 sets slower still.
+constexpr int kSignPF = 2;
 
 template <int I, int U, class F> __device__ __forceinline__ void sign_unroll(F &&f) {
   if constexpr (I < U) {
