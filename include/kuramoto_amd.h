@@ -191,6 +191,12 @@ int kmt_advance(kmt_ctx *ctx, int max_attempts, kmt_status *st);
  * (N values each, if keep_rows) and order parameters (step_eval,
  * kuramoto.cu:212-249).  Either pointer may be NULL. */
 int kmt_read_outputs(kmt_ctx *ctx, int max_rows, void *theta_rows, void *r, int *n_rows);
+/* kmt_read_outputs plus the couplings (C_i, S_i) of every drained row, 2N
+ * values a row, interleaved as kmt_rhs writes them (step_eval's
+ * makecoupling for couplings.dat, kuramoto.cu:226-231), computed on the
+ * device from the row still in the ring: no host round trip of the phases.
+ * Needs keep_rows; equal bit for bit to kmt_rhs on the returned rows. */
+int kmt_read_outputs_couplings(kmt_ctx *ctx, int max_rows, void *theta_rows, void *r, void *couplings, int *n_rows);
 /* Drain the accepted-step log: t and h after each accepted step
  * (times.dat, kuramoto.cu:181-188, and the verbose line's h). */
 int kmt_read_times(kmt_ctx *ctx, int max_n, void *t, void *h, int *n);

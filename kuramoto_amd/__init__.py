@@ -88,6 +88,7 @@ _SIGNATURES = {
     "kmt_begin": ([_P, _P, _I, _D, _I, _I], _I),
     "kmt_advance": ([_P, _I, _P], _I),
     "kmt_read_outputs": ([_P, _I, _P, _P, _P], _I),
+    "kmt_read_outputs_couplings": ([_P, _I, _P, _P, _P, _P], _I),
     "kmt_read_times": ([_P, _I, _P, _P, _P], _I),
     "kmt_step": ([_P, _P, _P, _P], _I),
     "kmt_eval": ([_P, _D, _P], _I),
@@ -326,6 +327,18 @@ class Integrator:
         n = ctypes.c_int()
         _check(self.lib.kmt_read_outputs(self.ctx, n_max, _ptr(rows), _ptr(r), ctypes.byref(n)), "kmt_read_outputs")
         return (rows[: n.value] if rows is not None else None), r[: n.value]
+
+    def read_outputs_couplings(self, max_rows=None):
+        """read_outputs plus each row's couplings (C_i, S_i), shape (n, 2N),
+        computed on the device from the rows in the ring (needs keep_rows)."""
+        n_max = self.ring_rows if max_rows is None else max_rows
+        rows = np.zeros((n_max, self.N), self.dtype)
+        r = np.zeros(n_max, self.dtype)
+        cpl = np.zeros((n_max, 2 * self.N), self.dtype)
+        n = ctypes.c_int()
+        _check(self.lib.kmt_read_outputs_couplings(self.ctx, n_max, _ptr(rows), _ptr(r), _ptr(cpl), ctypes.byref(n)),
+               "kmt_read_outputs_couplings")
+        return rows[: n.value], r[: n.value], cpl[: n.value]
 
     def read_times(self, max_n=4096):
         t = np.zeros(max_n, self.dtype)

@@ -342,7 +342,11 @@ int kmt_advance(kmt_ctx *ctx, int max_attempts, kmt_status *st) {
 }
 int kmt_read_outputs(kmt_ctx *ctx, int max_rows, void *theta_rows, void *r, int *n_rows) {
   CTX_CHECK;
-  return ctx->eng->read_outputs(max_rows, theta_rows, r, n_rows);
+  return ctx->eng->read_outputs(max_rows, theta_rows, r, nullptr, n_rows);
+}
+int kmt_read_outputs_couplings(kmt_ctx *ctx, int max_rows, void *theta_rows, void *r, void *couplings, int *n_rows) {
+  CTX_CHECK;
+  return ctx->eng->read_outputs(max_rows, theta_rows, r, couplings, n_rows);
 }
 int kmt_read_times(kmt_ctx *ctx, int max_n, void *t, void *h, int *n) {
   CTX_CHECK;

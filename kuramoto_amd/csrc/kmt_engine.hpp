@@ -47,7 +47,7 @@ struct EngineBase {
   virtual int order(const void *, double *) = 0;
   virtual int begin(const double *, int, double, int, int) = 0;
   virtual int advance(int, kmt_status *) = 0;
-  virtual int read_outputs(int, void *, void *, int *) = 0;
+  virtual int read_outputs(int, void *, void *, void *, int *) = 0;
   virtual int read_times(int, void *, void *, int *) = 0;
   virtual int step(int *, double *, double *) = 0;
   virtual int eval(double, void *) = 0;
@@ -1189,14 +1189,27 @@ template <class R> struct Engine : EngineBase {
     return KMT_OK;
   }
 
-  int read_outputs(int max_rows, void *rows, void *rv, int *nout) override {
+  // cpl (optional): the couplings (C_i, S_i) of every drained row, 2N values
+  // a row, computed from the row where it sits in the ring (the launches of
+  // kmt_rhs's coupling form, one row at a time through the shared (cos, sin)
+  // and pair buffers, each copied out before the next row overwrites them).
+  int read_outputs(int max_rows, void *rows, void *rv, void *cpl, int *nout) override {
     if (!begun) return fail(KMT_ERR_STATE, "kmt_read_outputs before kmt_begin");
     KMT_TRY(fetch_state());
     const int ready = shadow.ring_count;
     const int n = std::min(std::max(max_rows, 0), ready);
-    if (rows && n > 0) {
-      if (!keep_rows) return fail(KMT_ERR_STATE, "rows were not kept (kmt_begin keep_rows=0)");
+    if ((rows || cpl) && n > 0 && !keep_rows) return fail(KMT_ERR_STATE, "rows were not kept (kmt_begin keep_rows=0)");
+    if (cpl && n > 0 && !coupling_ready()) return fail(KMT_ERR_STATE, "no adjacency for the couplings");
+    if (rows && n > 0)
       HIP_TRY(hipMemcpy2DAsync(rows, sizeof(R) * N, ring, sizeof(R) * ldv, sizeof(R) * N, n, hipMemcpyDeviceToHost, stream));
+    for (int q = 0; cpl && q < n; ++q) {
+      sincos_kernel<R><<<(unsigned)((N + kBlock - 1) / kBlock), kBlock, 0, stream>>>(ring + (size_t)q * ldv, N, ldv, cs_scr);
+      Vecs<R> v = vecs(cs_scr, nullptr, 0);
+      v.f_out = f_scr;
+      v.cs_pairs = pairs;
+      launch_by_kind(KIND_COUPLING, v);
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipMemcpyAsync((R *)cpl + (size_t)q * 2 * N, pairs, sizeof(R) * 2 * N, hipMemcpyDeviceToHost, stream));
     }
     if (rv && n > 0) HIP_TRY(hipMemcpyAsync(rv, r_ring, sizeof(R) * n, hipMemcpyDeviceToHost, stream));
     // keep the unread rows at the front of the ring (in order)

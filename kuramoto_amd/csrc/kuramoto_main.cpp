@@ -473,7 +473,7 @@ int main(int argc, char *argv[]) {
   const double loop0 = now_s();
   kmt_status st;
   std::memset(&st, 0, sizeof(st));
-  std::vector<R> ts(4096), hs(4096), rv(ring_rows), cpl(2 * N);
+  std::vector<R> ts(4096), hs(4096), rv(ring_rows);
   Writer writer(keep_rows ? sizeof(R) * (size_t)ring_rows * N : 0, dense);
   long steps = 0;
   int chunk = 1;
@@ -505,16 +505,18 @@ int main(int argc, char *argv[]) {
     // dense-output rows and order parameters (kuramoto.cu:212-249)
     int nr = 0;
     if (keep_rows) b.rows = writer.take_rows();
-    CALL(kmt_read_outputs(g_ctx, ring_rows, b.rows, rv.data(), &nr));
+    if (dense >= 2) {  // couplings of each row from the row in device memory (kuramoto.cu:226-231)
+      const int want = std::min(st.rows_ready, ring_rows);  // rows the last advance left in the ring
+      b.cpl.resize((size_t)want * 2 * N);
+      CALL(kmt_read_outputs_couplings(g_ctx, want, b.rows, rv.data(), b.cpl.data(), &nr));
+      b.cpl.resize((size_t)nr * 2 * N);
+    } else {
+      CALL(kmt_read_outputs(g_ctx, ring_rows, b.rows, rv.data(), &nr));
+    }
     b.nrows = (size_t)nr;
     b.n_osc = N;
     if (nr > 0) {
-      if (dense >= 2) {
-        b.cpl.resize((size_t)nr * 2 * N);
-        for (int q = 0; q < nr; ++q)
-          CALL(kmt_rhs(g_ctx, b.rows + (size_t)q * N, nullptr, b.cpl.data() + (size_t)q * 2 * N));
-        floc.assign(b.cpl.end() - 2 * N, b.cpl.end());
-      }
+      if (dense >= 2) floc.assign(b.cpl.end() - 2 * N, b.cpl.end());
       b.r.assign(rv.begin(), rv.begin() + nr);
     }
     // restart file (kuramoto.cu:251-264; kuramoto_64.cu writes it always)

@@ -342,3 +342,80 @@ def test_restart_in_new_context_is_bit_exact(km, N, prec):
     y2, t2, h2, _ = run(y1, t1, h1, 7)
     assert rows == (4 if prec == "f32" else 8)
     assert t2 == t_full and h2 == h_full and np.array_equal(y2, y_full)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("mode", ["dense", "factored", "signs"])
+def test_drained_couplings_equal_rhs_on_rows(km, orc, prec, mode):
+    """kmt_read_outputs_couplings (couplings.dat's rows, kuramoto.cu:226-231)
+    computes each drained row's couplings on the device from the row in the
+    ring: equal bit for bit to kmt_rhs on the rows it returns, for the dense
+    matrix, the factored -A contraction and the packed-sign factor; the rows
+    and r equal kmt_read_outputs's; the last row's couplings match the
+    oracle's within the tolerance of the RHS parity tests (dense only)."""
+    p = km.F32 if prec == "f32" else km.F64
+    N, K, seed, c1, t1 = 700, 5, 6, 3.0, 0.5
+    atol = 1e-6 if prec == "f32" else 1e-9
+    dtype, theta0, omega, X, w = _inputs(km, p, N, K, seed, normal=True, init=6.283185307)
+    te = _grid(0.0, t1, 0.05, dtype)
+    print(f"case drained couplings mode={mode} prec={prec}", flush=True)
+    outs = {}
+    for drain in ("plain", "couplings"):
+        with km.Integrator(N, p, atol=atol, c1=c1) as it:
+            it.set_omega(omega)
+            if mode == "signs":
+                it.set_factor_sign(N, seed)
+                it.use_factored(True)
+            else:
+                it.set_factor(X, w)
+                if mode == "dense":
+                    it.build_adjacency()
+                else:
+                    it.use_factored(True)
+            it.set_state(theta0, 0.0, 0.01)
+            it.begin(te, t1, keep_rows=True, ring_rows=4)
+            rows, rs, cps = [], [], []
+            while True:
+                st = it.advance(3)
+                it.read_times()
+                if drain == "plain":
+                    r_rows, r = it.read_outputs()
+                else:
+                    r_rows, r, cp = it.read_outputs_couplings()
+                    cps.append(cp)
+                rows.append(r_rows)
+                rs.append(r)
+                if not st.active:
+                    break
+            rows, rs = np.concatenate(rows), np.concatenate(rs)
+            assert len(rs) == len(te)
+            if drain == "couplings":
+                cps = np.concatenate(cps)
+                assert cps.shape == (len(te), 2 * N)
+                for q in range(len(te)):
+                    assert np.array_equal(cps[q], it.couplings(rows[q])), q
+            outs[drain] = (rows, rs)
+    assert np.array_equal(outs["plain"][0], outs["couplings"][0])
+    assert np.array_equal(outs["plain"][1], outs["couplings"][1])
+    if mode == "dense":
+        M = orc.adj_rows(X, w, 0, N).astype(dtype)
+        _, cso = orc.rhs(orc.Adj(0, M=M), omega, c1, rows[-1], dtype=dtype)
+        rsum = np.abs(M.astype(np.float64)).sum(axis=1)
+        tolc = (1.2e-7 if prec == "f32" else 1e-12) * np.repeat(rsum, 2)
+        assert np.all(np.abs(cps[-1].astype(np.float64) - cso) <= tolc + 1e-30)
+
+
+def test_drained_couplings_need_kept_rows(km):
+    N, K, seed = 300, 5, 2
+    dtype, theta0, omega, X, w = _inputs(km, km.F64, N, K, seed, normal=True, init=1.0)
+    te = _grid(0.0, 0.2, 0.05, dtype)
+    with km.Integrator(N, km.F64, atol=1e-8) as it:
+        it.set_omega(omega)
+        it.set_factor(X, w)
+        it.build_adjacency()
+        it.set_state(theta0, 0.0, 0.01)
+        it.begin(te, 0.2, keep_rows=False, ring_rows=8)
+        st = it.advance(64)
+        assert st.rows_ready > 0
+        with pytest.raises(km.KmtError, match="KMT_ERR_STATE"):
+            it.read_outputs_couplings()
