@@ -34,6 +34,12 @@
 //   --max-steps=n       stop after n accepted steps, leaving the files as a
 //                       process killed at that point would (no final block)
 //   --chunk=n           maximum step attempts per host synchronisation
+//   --exact-final       final fs.dat holds the integrator's state (theta at
+//                       its t, that t and h, the couplings of that theta),
+//                       as the per-step restart file does, instead of the
+//                       reference's interpolation with k1 already replaced
+//                       by k7 beside the integrator's t (kuramoto.cu:630-647,
+//                       dp45.cu:158-161): -R then continues the run exactly
 #include <getopt.h>
 #include <sys/time.h>
 #include <unistd.h>
@@ -103,7 +109,7 @@ static void usage() {
   std::printf("atol is absolute error tolerance. Default 1e-6.\n");
   std::printf("gpu is index of the gpu. Default 0.\n");
   std::printf("filebase is base file name for output. \n");
-  std::printf("extensions: --gauss --adj-dtype=native|bf16|f16|sym|sym-bf16|sym-f16 --rebuild --max-steps=n --chunk=n\n");
+  std::printf("extensions: --gauss --adj-dtype=native|bf16|f16|sym|sym-bf16|sym-f16 --rebuild --max-steps=n --chunk=n --exact-final\n");
 }
 
 static void append(const char *suffix, const void *data, size_t elem, size_t n) {
@@ -224,7 +230,7 @@ int main(int argc, char *argv[]) {
   const double start = now_s();
   R c1 = 3.0, t1 = 1e2, dt = 1e0, freq = 1.0, atl = 1e-6, rtl = 0, I = 0;
   int gpu = 0, seed = 1, fixed = 0, A = 0, verbose = 0, help = 1, dense = 3, normal = 0, reload = 0;
-  int gauss = 0, storage = KMT_ADJ_NATIVE, max_steps = -1, chunk_max = 64, rebuild = 0;
+  int gauss = 0, storage = KMT_ADJ_NATIVE, max_steps = -1, chunk_max = 64, rebuild = 0, exact_final = 0;
   unsigned long N = 5000, K = 6;
   const char *filebase = nullptr;
   static struct option longopts[] = {{"gauss", no_argument, nullptr, 1001},
@@ -232,6 +238,7 @@ int main(int argc, char *argv[]) {
                                      {"max-steps", required_argument, nullptr, 1003},
                                      {"chunk", required_argument, nullptr, 1004},
                                      {"rebuild", no_argument, nullptr, 1005},
+                                     {"exact-final", no_argument, nullptr, 1006},
                                      {nullptr, 0, nullptr, 0}};
   // kuramoto.cu:298-362 (options may follow the positional filebase)
   while (optind < argc) {
@@ -269,6 +276,7 @@ int main(int argc, char *argv[]) {
         case 1003: max_steps = atoi(optarg); break;
         case 1004: chunk_max = std::max(1, atoi(optarg)); break;
         case 1005: rebuild = 1; break;
+        case 1006: exact_final = 1; break;
         default: break;
       }
     } else {
@@ -544,11 +552,12 @@ int main(int argc, char *argv[]) {
   }
 
   // final state with couplings, kuramoto.cu:630-647: interpolation over the
-  // last step with k1 already replaced by k7, integrator t and h
+  // last step with k1 already replaced by k7, integrator t and h; with
+  // --exact-final the integrator's own state at that t
   std::vector<R> yev(N);
   double tt = 0, hh = 0;
   CALL(kmt_get_state(g_ctx, yev.data(), &tt, &hh));
-  if (started && st.accepted > 0 && n_eval > 0) CALL(kmt_eval(g_ctx, t_eval_d[n_eval - 1], yev.data()));
+  if (!exact_final && started && st.accepted > 0 && n_eval > 0) CALL(kmt_eval(g_ctx, t_eval_d[n_eval - 1], yev.data()));
   CALL(kmt_rhs(g_ctx, yev.data(), nullptr, floc.data()));
   write_fs(yev, (R)tt, (R)hh, floc);
 

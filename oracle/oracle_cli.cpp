@@ -14,6 +14,9 @@
  *                  every RHS instead of storing N^2 values.
  *   --max-steps n  stop after n accepted steps and leave the files as a
  *                  process killed at that point would (no final block).
+ *   --exact-final  final fs.dat holds the integrator's state (y at its t,
+ *                  t, h, couplings of y) instead of the reference's
+ *                  interpolation with k1 already replaced by k7.
  * A summary line "oracle: ..." with loop time and osc*steps/s goes to stdout.
  */
 #include <getopt.h>
@@ -121,19 +124,20 @@ static void usage() {
   std::printf("seed is random seed. Default 1. \ninit is uniform random initial condition scale. Default 0. \n");
   std::printf("rtol is relative error tolerance. Default 0.\natol is absolute error tolerance. Default 1e-6.\n");
   std::printf("gpu is index of the gpu. Default 0.\nfilebase is base file name for output. \n");
-  std::printf("oracle extensions: --gauss --rebuild --max-steps n\n");
+  std::printf("oracle extensions: --gauss --rebuild --max-steps n --exact-final\n");
 }
 
 int main(int argc, char *argv[]) {
   double start = now_s();
   R c1 = 3.0, t1 = 1e2, dt = 1e0, freq = 1.0, atl = 1e-6, rtl = 0, I = 0;
   int seed = 1, fixed = 0, A = 0, verbose = 0, help = 1, dense = 3, normal = 0, reload = 0;
-  int gauss = 0, rebuild = 0, max_steps = -1;
+  int gauss = 0, rebuild = 0, max_steps = -1, exact_final = 0;
   long N = 5000, K = 6;
   const char *filebase = nullptr;
   static struct option longopts[] = {{"gauss", no_argument, nullptr, 1001},
                                      {"rebuild", no_argument, nullptr, 1002},
                                      {"max-steps", required_argument, nullptr, 1003},
+                                     {"exact-final", no_argument, nullptr, 1004},
                                      {nullptr, 0, nullptr, 0}};
   while (optind < argc) {
     int c = getopt_long(argc, argv, "N:K:I:D:c:g:t:d:f:s:r:a:hvFnRA", longopts, nullptr);
@@ -160,6 +164,7 @@ int main(int argc, char *argv[]) {
         case 1001: gauss = 1; break;
         case 1002: rebuild = 1; break;
         case 1003: max_steps = atoi(optarg); break;
+        case 1004: exact_final = 1; break;
       }
     } else {
       filebase = argv[optind];
@@ -351,7 +356,7 @@ int main(int argc, char *argv[]) {
 
   /* final block, kuramoto.cu:630-647: interpolation uses k1 already replaced
    * by k7 and the integrator t */
-  if (n_eval > 0) dp.eval(t, p.t_eval[n_eval - 1], p.yeval.data());
+  if (n_eval > 0 && !exact_final) dp.eval(t, p.t_eval[n_eval - 1], p.yeval.data());
   else p.yeval = dp.y;
   model.sincos(p.yeval.data());
   model.coupling();

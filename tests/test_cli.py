@@ -75,6 +75,36 @@ def test_oracle_cli_kill_resume(tmp_path, orc):
         assert open(a + suf, "rb").read() == open(b + suf, "rb").read(), suf
 
 
+def check_exact_final_continuation(prog, tmp_path, dt):
+    """--exact-final (the optional fix-up of the reference's final fs.dat,
+    kuramoto.cu:630-647 with dp45.cu:158-161): the last file holds the
+    integrator's state, so -R with a later -t continues the run exactly: the
+    accepted-step times, the grid rows (order, thetas, couplings) and the
+    final fs.dat equal one uninterrupted run's byte for byte.  Without the
+    flag the final fs.dat holds a phase vector interpolated at t_eval[last]
+    beside a later t."""
+    a, b, q = str(tmp_path / "a"), str(tmp_path / "b"), str(tmp_path / "q")
+    base = ["-N", "120", "-K", "5", "-c", "3", "-d", "0.25", "-s", "3", "-I", "6.283185307"]
+    run(prog, base + ["-t", "2", "--exact-final"], a)
+    run(prog, base + ["-t", "2"], q)
+    s = np.dtype(dt).itemsize
+    fa, fq = load(a, "fs.dat", dt), load(q, "fs.dat", dt)
+    t_a = float(fa[120])
+    assert 2.0 <= t_a < 2.25  # the continuation's grid starts where the first run's ended
+    assert fa[120] == fq[120] and fa[121] == fq[121]  # same integrator t and h
+    assert not np.array_equal(fa[:120], fq[:120])  # the quirk: another phase vector
+    n_first = 1 + 8
+    run(prog, base + ["-t", "3", "--exact-final", "-R"], a)
+    run(prog, base + ["-t", "3", "--exact-final"], b)
+    for suf in ("times.dat", "fs.dat", "order.dat", "thetas.dat", "couplings.dat"):
+        assert open(a + suf, "rb").read() == open(b + suf, "rb").read(), suf
+    assert os.path.getsize(a + "order.dat") == (n_first + 4) * s
+
+
+def test_oracle_cli_exact_final_continues(tmp_path, orc):
+    check_exact_final_continuation(os.path.join(ORC, "kuramoto_oracle_64"), tmp_path, np.float64)
+
+
 # ------------------------------------------------------------- GPU: product
 @pytest.mark.gpu
 @pytest.mark.parametrize("prec", ["f32", "f64"])
@@ -331,7 +361,7 @@ def test_cli_usage_exits_without_running(tmp_path, orc, prog, args):
     assert lines[-2] == ref_lines[-2] == "filebase is base file name for output. "
     assert lines[:-1] == ref_lines[:-1] and len(lines) == 24
     assert lines[-1] == ("extensions: --gauss --adj-dtype=native|bf16|f16|sym|sym-bf16|sym-f16 --rebuild "
-                         "--max-steps=n --chunk=n")
+                         "--max-steps=n --chunk=n --exact-final")
     assert os.listdir(tmp_path) == []
 
 
@@ -459,3 +489,10 @@ def test_cli_reference_quirks_restart_and_overshoot(tmp_path, km, orc):
     # CLI comparison above allows 2% in the times): both overshoot t1
     assert to[-1] >= np.float32(1.0) and abs(len(ta) - len(to)) <= 1
     assert abs(float(ta[-1]) - float(to[-1])) <= 0.02 * float(to[-1])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prog_name,dt", [("kuramoto", np.float32), ("kuramoto_64", np.float64)])
+def test_cli_exact_final_continues(tmp_path, km, prog_name, dt):
+    print(f"case cli exact-final {prog_name}", flush=True)
+    check_exact_final_continuation(os.path.join(BIN, prog_name), tmp_path, dt)
