@@ -12,7 +12,8 @@
 //                      against (cos, sin) of its columns -> partials of rows
 //                      in block row I, and (off the diagonal) column sums
 //                      against (cos, sin) of its rows -> partials of rows in
-//                      block row J (M_ji = M_ij);
+//                      block row J (M_ji = M_ij); sym_span_kernel does the
+//                      same over a span of blocks per resident workgroup;
 //   sym_reduce_kernel  one thread per row i (block row K): the nb partials
 //                      of row i summed in the order of the other block index
 //                      (column-sum partials of blocks (L, K), L < K, then
@@ -46,6 +47,7 @@ template <class R, class S> struct SymArgs {
   const S *A;          // stored blocks
   const int2 *ij;      // (I, J) of every stored block
   long nb;             // blocks per row / column of the padded matrix
+  long nblocks;        // stored blocks, nb (nb + 1) / 2
   SymPair<Acc> *prow;  // [block][kSymT]: row sums (C, S) of the block
   SymPair<Acc> *pcol;  // [block][kSymT]: column sums of off-diagonal blocks
   R mscale;            // M = mscale * stored
@@ -124,67 +126,54 @@ template <class R> __device__ __forceinline__ R wave_reduce8(R (&v)[8], int lane
   return s;
 }
 
-// Wave w of the workgroup owns rows w, w + 4, w + 8, ... of the block, four
-// at a time with the next four in flight, so that the workgroup's loads at
-// any moment cover one contiguous stretch of the block (6% faster than four
-// separate quarter-blocks per workgroup); lane l owns the columns
-// q 64 VEC + l VEC + e (q < NLD, e < VEC): four per lane, loaded 16 bytes
-// at a time, coalesced per row.
-template <class R, class S>
-__global__ __launch_bounds__(kBlock) void sym_tile_kernel(const SymArgs<R, S> a) {
-  if (a.v.check_state && (!a.v.st->active || a.v.st->commit_pending)) return;
+// Shape of the block pass: wave w of the workgroup owns rows w, w + 4,
+// w + 8, ... of the block, RB at a time with the next RB in flight, so that
+// the workgroup's loads at any moment cover one contiguous stretch of the
+// block (6% faster than four separate quarter-blocks per workgroup); lane l
+// owns the columns q 64 VEC + l VEC + e (q < NLD, e < VEC): four per lane,
+// loaded 16 (8) bytes at a time, coalesced per row.
+template <class R, class S> struct SymPass {
   using Acc = typename Store<S>::Acc;
   using Val = typename Store<S>::Val;
   using Raw = typename SymLoad<S>::Raw;
-  constexpr int VEC = SymLoad<S>::VEC;
-  constexpr int NLD = kSymT / (kWave * VEC);
-  constexpr int NC = NLD * VEC;
-  constexpr int RB = 4;
-  constexpr int NW = kBlock / kWave;
-  constexpr int RW = kSymT / NW;
+  static constexpr int VEC = SymLoad<S>::VEC;
+  static constexpr int NLD = kSymT / (kWave * VEC);
+  static constexpr int NC = NLD * VEC;
+  static constexpr int RB = 4;
+  static constexpr int NW = kBlock / kWave;
+  static constexpr int RW = kSymT / NW;
   static_assert(NC == 4 && RW % (2 * RB) == 0, "block shape");
-  __shared__ Acc vi[2][kSymT];
-  __shared__ Acc colbuf[NW][2][kSymT];
-  const int tid = threadIdx.x, lane = tid % kWave, wave = tid / kWave;
-  const long blk = blockIdx.x;
-  // the block's rows depend on the block index only: the first batch is
-  // issued before the block's coordinates and vectors are known
-  const S *base = a.A + (size_t)blk * kSymT * kSymT + lane * VEC;
-  auto row_of = [&](int k) { return wave + NW * k; };  // the wave's k-th row
-  auto load = [&](Raw (&raw)[RB][NLD], int kb) {
+
+  __device__ static int row_of(int wave, int k) { return wave + NW * k; }  // the wave's k-th row
+
+  // The block as a buffer resource: its base in scalar registers, so a
+  // lane's address is one 32-bit offset (voff, the lane's row-0 columns in
+  // the wave's first row) and each load's row is a scalar constant; the
+  // hardware returns 0 past the block's bytes.
+  __device__ static __amdgpu_buffer_rsrc_t block_rsrc(const S *blk_base) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<S *>(blk_base), (short)0, (int)(kSymT * kSymT * sizeof(S)),
+                                            0x00020000);
+  }
+  __device__ static int lane_offset(int wave, int lane) { return (wave * kSymT + lane * VEC) * (int)sizeof(S); }
+
+  // rows kb .. kb + RB - 1 of the wave (lane's columns), non-temporal
+  __device__ static void load(Raw (&raw)[RB][NLD], __amdgpu_buffer_rsrc_t rs, int voff, int kb) {
 #pragma unroll
     for (int r = 0; r < RB; ++r)
 #pragma unroll
-      for (int q = 0; q < NLD; ++q)
-        raw[r][q] = __builtin_nontemporal_load(
-            reinterpret_cast<const Raw *>(base + (size_t)row_of(kb + r) * kSymT + q * kWave * VEC));
-  };
-  Raw rawA[RB][NLD], rawB[RB][NLD];
-  load(rawA, 0);
-  const int2 ij = a.ij[blk];
-  const long I = ij.x, J = ij.y, N = a.v.N;
-  const R *cosv = a.v.cs_in, *sinv = a.v.cs_in + a.v.ldv;
-  {
-    const long i = I * kSymT + tid;
-    vi[0][tid] = i < N ? (Acc)cosv[i] : (Acc)0;
-    vi[1][tid] = i < N ? (Acc)sinv[i] : (Acc)0;
+      for (int q = 0; q < NLD; ++q) {
+        const int soff = (NW * (kb + r) * kSymT + q * kWave * VEC) * (int)sizeof(S);
+        if constexpr (sizeof(Raw) == 16) raw[r][q] = __builtin_amdgcn_raw_buffer_load_b128(rs, voff, soff, 2);
+        else raw[r][q] = __builtin_amdgcn_raw_buffer_load_b64(rs, voff, soff, 2);
+      }
   }
-  Acc cj[NC], sj[NC];
-#pragma unroll
-  for (int q = 0; q < NLD; ++q)
-#pragma unroll
-    for (int e = 0; e < VEC; ++e) {
-      const long j = J * kSymT + q * kWave * VEC + lane * VEC + e;
-      cj[q * VEC + e] = j < N ? (Acc)cosv[j] : (Acc)0;
-      sj[q * VEC + e] = j < N ? (Acc)sinv[j] : (Acc)0;
-    }
-  __syncthreads();
-  Acc colC[NC], colS[NC];
-#pragma unroll
-  for (int k = 0; k < NC; ++k) { colC[k] = 0; colS[k] = 0; }
-  // the wave's rows kb .. kb + RB - 1: row sums reduced over the wave,
-  // column sums kept per lane
-  auto process = [&](const Raw (&raw)[RB][NLD], int kb) {
+
+  // The wave's rows kb .. kb + RB - 1: row sums against the columns'
+  // (cos, sin) reduced over the wave and stored to the block's row partials
+  // prow_blk; column sums against the rows' (cos, sin) (vi) kept per lane.
+  __device__ static void process(const Raw (&raw)[RB][NLD], int kb, int wave, int lane, const Acc (&vi)[2][kSymT],
+                                 const Acc (&cj)[NC], const Acc (&sj)[NC], Acc (&colC)[NC], Acc (&colS)[NC],
+                                 SymPair<Acc> *prow_blk) {
     Acc part[2 * RB];
 #pragma unroll
     for (int r = 0; r < RB; ++r) {
@@ -194,7 +183,7 @@ __global__ __launch_bounds__(kBlock) void sym_tile_kernel(const SymArgs<R, S> a)
       Acc m[NC];
 #pragma unroll
       for (int k = 0; k < NC; ++k) m[k] = (Acc)mv[k];
-      const int row = row_of(kb + r);
+      const int row = row_of(wave, kb + r);
       const Acc ci = vi[0][row], si = vi[1][row];
       Acc pc = m[0] * cj[0], ps = m[0] * sj[0];
 #pragma unroll
@@ -210,40 +199,185 @@ __global__ __launch_bounds__(kBlock) void sym_tile_kernel(const SymArgs<R, S> a)
       part[r] = pc;
       part[RB + r] = ps;
     }
+    // every lane stores its slot's sum (the 8 lanes of a slot hold the same
+    // value): no divergent branch around the store
     const Acc tot = wave_reduce8<Acc>(part, lane);
-    if ((lane & 7) == 0) {
-      const int slot = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
-      Acc *dst = reinterpret_cast<Acc *>(a.prow + (size_t)blk * kSymT + row_of(kb + (slot % RB)));
-      dst[slot / RB] = tot;
-    }
-  };
-  // two row batches in flight: the next one loads while this one is reduced
-#pragma unroll 1
-  for (int kb = 0; kb < RW; kb += 2 * RB) {
-    load(rawB, kb + RB);
-    process(rawA, kb);
-    if (kb + 2 * RB < RW) load(rawA, kb + 2 * RB);
-    process(rawB, kb + RB);
+    const int slot = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
+    Acc *dst = reinterpret_cast<Acc *>(prow_blk + row_of(wave, kb + (slot % RB)));
+    dst[slot / RB] = tot;
   }
-  if (I == J) return;  // the diagonal block is stored whole: its row sums are complete
+
+  // (cos, sin) at index i, zero past N: the load is clamped into [0, N)
+  // rather than branched around, so no divergent branch separates loads
+  // (a branch around a load makes the compiler wait for every load in
+  // flight where the paths join)
+  __device__ static void vec_at(const R *cosv, const R *sinv, long i, long N, Acc &c, Acc &s) {
+    const long k = i < N ? i : N - 1;
+    const Acc cv = (Acc)cosv[k], sv = (Acc)sinv[k];
+    c = i < N ? cv : (Acc)0;
+    s = i < N ? sv : (Acc)0;
+  }
+
+  // (cos, sin) of the lane's columns of block column J (zero past N)
+  __device__ static void col_vectors(const R *cosv, const R *sinv, long J, long N, int lane, Acc (&cj)[NC],
+                                     Acc (&sj)[NC]) {
 #pragma unroll
-  for (int q = 0; q < NLD; ++q)
+    for (int q = 0; q < NLD; ++q)
 #pragma unroll
-    for (int e = 0; e < VEC; ++e) {
-      const int c = q * kWave * VEC + lane * VEC + e;
-      colbuf[wave][0][c] = colC[q * VEC + e];
-      colbuf[wave][1][c] = colS[q * VEC + e];
+      for (int e = 0; e < VEC; ++e)
+        vec_at(cosv, sinv, J * kSymT + q * kWave * VEC + lane * VEC + e, N, cj[q * VEC + e], sj[q * VEC + e]);
+  }
+
+  // column sums of the block over the NW waves (fixed order) -> pcol_blk
+  __device__ static void finish_columns(Acc (&colbuf)[NW][2][kSymT], const Acc (&colC)[NC], const Acc (&colS)[NC],
+                                        int wave, int lane, int tid, SymPair<Acc> *pcol_blk) {
+#pragma unroll
+    for (int q = 0; q < NLD; ++q)
+#pragma unroll
+      for (int e = 0; e < VEC; ++e) {
+        const int c = q * kWave * VEC + lane * VEC + e;
+        colbuf[wave][0][c] = colC[q * VEC + e];
+        colbuf[wave][1][c] = colS[q * VEC + e];
+      }
+    __syncthreads();
+    SymPair<Acc> out;
+    out.c = colbuf[0][0][tid];
+    out.s = colbuf[0][1][tid];
+#pragma unroll
+    for (int w = 1; w < NW; ++w) {
+      out.c += colbuf[w][0][tid];
+      out.s += colbuf[w][1][tid];
     }
+    pcol_blk[tid] = out;
+  }
+};
+
+// One workgroup per stored block.
+template <class R, class S>
+__global__ __launch_bounds__(kBlock) void sym_tile_kernel(const SymArgs<R, S> a) {
+  if (a.v.check_state && (!a.v.st->active || a.v.st->commit_pending)) return;
+  using P = SymPass<R, S>;
+  using Acc = typename P::Acc;
+  using Raw = typename P::Raw;
+  constexpr int RB = P::RB, NLD = P::NLD, NC = P::NC, NW = P::NW, RW = P::RW;
+  __shared__ Acc vi[2][kSymT];
+  __shared__ Acc colbuf[NW][2][kSymT];
+  const int tid = threadIdx.x, lane = tid % kWave, wave = tid / kWave;
+  const long blk = blockIdx.x;
+  // the block's rows depend on the block index only: the first two batches
+  // are issued before the block's coordinates and vectors are known
+  const __amdgpu_buffer_rsrc_t rs = P::block_rsrc(a.A + (size_t)blk * kSymT * kSymT);
+  const int voff = P::lane_offset(wave, lane);
+  Raw raw0[RB][NLD], raw1[RB][NLD];
+  P::load(raw0, rs, voff, 0);
+  P::load(raw1, rs, voff, RB);
+  const int2 ij = a.ij[blk];
+  const long I = ij.x, J = ij.y, N = a.v.N;
+  const R *cosv = a.v.cs_in, *sinv = a.v.cs_in + a.v.ldv;
+  {
+    Acc c, s;
+    P::vec_at(cosv, sinv, I * kSymT + tid, N, c, s);
+    vi[0][tid] = c;
+    vi[1][tid] = s;
+  }
+  Acc cj[NC], sj[NC];
+  P::col_vectors(cosv, sinv, J, N, lane, cj, sj);
   __syncthreads();
-  SymPair<Acc> out;
-  out.c = colbuf[0][0][tid];
-  out.s = colbuf[0][1][tid];
+  Acc colC[NC], colS[NC];
 #pragma unroll
-  for (int w = 1; w < NW; ++w) {
-    out.c += colbuf[w][0][tid];
-    out.s += colbuf[w][1][tid];
+  for (int k = 0; k < NC; ++k) { colC[k] = 0; colS[k] = 0; }
+  SymPair<Acc> *prow = a.prow + (size_t)blk * kSymT;
+  // a ring of two row batches: batch k + 2 loads into batch k's registers
+  // as soon as batch k is reduced, in the order the batches are used.  The
+  // loads in the loop are unconditional (the last pair is peeled off), so
+  // each batch stays in its own registers across the back edge: a
+  // conditional load there becomes a register copy that waits for the
+  // batch just issued, and the ring drains every iteration.
+#pragma unroll 1
+  for (int kb = 0; kb < RW - 2 * RB; kb += 2 * RB) {
+    P::process(raw0, kb, wave, lane, vi, cj, sj, colC, colS, prow);
+    P::load(raw0, rs, voff, kb + 2 * RB);
+    P::process(raw1, kb + RB, wave, lane, vi, cj, sj, colC, colS, prow);
+    P::load(raw1, rs, voff, kb + 3 * RB);
   }
-  a.pcol[(size_t)blk * kSymT + tid] = out;
+  P::process(raw0, RW - 2 * RB, wave, lane, vi, cj, sj, colC, colS, prow);
+  P::process(raw1, RW - RB, wave, lane, vi, cj, sj, colC, colS, prow);
+  if (I == J) return;  // the diagonal block is stored whole: its row sums are complete
+  P::finish_columns(colbuf, colC, colS, wave, lane, tid, a.pcol + (size_t)blk * kSymT);
+}
+
+// The same block pass over a contiguous span of stored blocks per workgroup
+// (one resident round: workgroup g takes blocks [g B / G, (g + 1) B / G)),
+// pipelined across block boundaries: while a block's last two batches are
+// reduced, the next block's first two batches and its (cos, sin) vectors
+// are already loading, so a workgroup's stream of loads does not stop at
+// every block for the block's set-up and column reduction.  Every block is
+// summed exactly as sym_tile_kernel sums it: the partials are the same bits.
+template <class R, class S>
+__global__ __launch_bounds__(kBlock) void sym_span_kernel(const SymArgs<R, S> a) {
+  if (a.v.check_state && (!a.v.st->active || a.v.st->commit_pending)) return;
+  using P = SymPass<R, S>;
+  using Acc = typename P::Acc;
+  using Raw = typename P::Raw;
+  constexpr int RB = P::RB, NLD = P::NLD, NC = P::NC, NW = P::NW, RW = P::RW;
+  __shared__ Acc vi[2][kSymT];
+  __shared__ Acc colbuf[NW][2][kSymT];
+  const int tid = threadIdx.x, lane = tid % kWave, wave = tid / kWave;
+  const long nblk = a.nblocks;
+  const long b0 = (long)blockIdx.x * nblk / gridDim.x, b1 = (long)(blockIdx.x + 1) * nblk / gridDim.x;
+  if (b0 >= b1) return;
+  const long N = a.v.N;
+  const R *cosv = a.v.cs_in, *sinv = a.v.cs_in + a.v.ldv;
+  const int voff = P::lane_offset(wave, lane);
+  __amdgpu_buffer_rsrc_t rs = P::block_rsrc(a.A + (size_t)b0 * kSymT * kSymT);
+  Raw raw0[RB][NLD], raw1[RB][NLD];
+  P::load(raw0, rs, voff, 0);
+  P::load(raw1, rs, voff, RB);
+  int2 ij = a.ij[b0];
+  Acc vic, vis, cj[NC], sj[NC];
+  P::vec_at(cosv, sinv, (long)ij.x * kSymT + tid, N, vic, vis);
+  P::col_vectors(cosv, sinv, ij.y, N, lane, cj, sj);
+#pragma unroll 1
+  for (long blk = b0; blk < b1; ++blk) {
+    // the next block, or (after the span's last block) a resource of no
+    // bytes: its loads return zeros without touching memory
+    const bool next = blk + 1 < b1;
+    const long nb = next ? blk + 1 : blk;
+    const int2 ij_next = a.ij[nb];
+    const __amdgpu_buffer_rsrc_t rsn = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<S *>(a.A + (size_t)nb * kSymT * kSymT), (short)0,
+        next ? (int)(kSymT * kSymT * sizeof(S)) : 0, 0x00020000);
+    const long I = ij.x, J = ij.y;
+    if (blk > b0) __syncthreads();  // the previous block's reads of vi and colbuf are done
+    vi[0][tid] = vic;
+    vi[1][tid] = vis;
+    __syncthreads();
+    Acc colC[NC], colS[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) { colC[k] = 0; colS[k] = 0; }
+    SymPair<Acc> *prow = a.prow + (size_t)blk * kSymT;
+#pragma unroll 1
+    for (int kb = 0; kb < RW - 2 * RB; kb += 2 * RB) {
+      P::process(raw0, kb, wave, lane, vi, cj, sj, colC, colS, prow);
+      P::load(raw0, rs, voff, kb + 2 * RB);
+      P::process(raw1, kb + RB, wave, lane, vi, cj, sj, colC, colS, prow);
+      P::load(raw1, rs, voff, kb + 3 * RB);
+    }
+    // the last pair; the next block's vectors and first two batches load
+    Acc ncj[NC], nsj[NC];
+    P::process(raw0, RW - 2 * RB, wave, lane, vi, cj, sj, colC, colS, prow);
+    P::vec_at(cosv, sinv, (long)ij_next.x * kSymT + tid, N, vic, vis);
+    P::col_vectors(cosv, sinv, ij_next.y, N, lane, ncj, nsj);
+    P::load(raw0, rsn, voff, 0);
+    P::process(raw1, RW - RB, wave, lane, vi, cj, sj, colC, colS, prow);
+    P::load(raw1, rsn, voff, RB);
+    if (I != J)  // the diagonal block is stored whole: its row sums are complete
+      P::finish_columns(colbuf, colC, colS, wave, lane, tid, a.pcol + (size_t)blk * kSymT);
+#pragma unroll
+    for (int k = 0; k < NC; ++k) { cj[k] = ncj[k]; sj[k] = nsj[k]; }
+    ij = ij_next;
+    rs = rsn;
+  }
 }
 
 // One thread per row; a workgroup covers exactly one block row (kBlock =
