@@ -119,7 +119,6 @@ template <class R> struct Engine : EngineBase {
   int tune_rows = 8, tune_nt = 1, tune_bal = 0, tuned = 0;
   double tune_ms[12] = {0};
   int bal_grid = 0;  // workgroups of the balanced variant (one resident round)
-  int sym_grid = 0, sym_grid_storage = -1;  // the same for sym_span_kernel
   R t0_ = 0, h0_ = 0;
 
   ~Engine() override { release(); }
@@ -240,7 +239,6 @@ template <class R> struct Engine : EngineBase {
     tune_nt = flags & 1;
     tune_bal = (flags >> 1) & 1;
     bal_grid = 0;
-    sym_grid = 0;
     tuned = 1;
     drop_graphs();
     return KMT_OK;
@@ -803,24 +801,8 @@ template <class R> struct Engine : EngineBase {
     a.pcol = reinterpret_cast<SymPair<Acc> *>(sym_pcol);
     a.mscale = (R)mscale_;
     a.v = v;
-    a.nblocks = sym_blocks;
-    if (tune_bal) {
-      if (sym_grid <= 0 || sym_grid_storage != storage) {
-        sym_grid = span_grid(sym_span_kernel<R, S>, sym_blocks);
-        sym_grid_storage = storage;
-      }
-      sym_span_kernel<R, S><<<(unsigned)sym_grid, kBlock, 0, stream>>>(a);
-    } else {
-      sym_tile_kernel<R, S><<<(unsigned)sym_blocks, kBlock, 0, stream>>>(a);
-    }
+    sym_tile_kernel<R, S><<<(unsigned)sym_blocks, kBlock, 0, stream>>>(a);
     sym_reduce_kernel<R, S, KIND><<<(unsigned)sym_nb, kBlock, 0, stream>>>(a);
-  }
-  // workgroups of one resident round of a kernel (<= units)
-  template <class K> int span_grid(K kernel, long units) {
-    int per_cu = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0) != hipSuccess || per_cu < 1) per_cu = 1;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus < 1) cus = 1;
-    return (int)std::max<long>(1, std::min<long>((long)per_cu * cus, units));
   }
   template <int KIND> void launch_sym(const Vecs<R> &v) {
     if (storage == KMT_ADJ_SYM) launch_sym_as<R, KIND>(v);

@@ -12,8 +12,7 @@
 //                      against (cos, sin) of its columns -> partials of rows
 //                      in block row I, and (off the diagonal) column sums
 //                      against (cos, sin) of its rows -> partials of rows in
-//                      block row J (M_ji = M_ij); sym_span_kernel does the
-//                      same over a span of blocks per resident workgroup;
+//                      block row J (M_ji = M_ij);
 //   sym_reduce_kernel  one thread per row i (block row K): the nb partials
 //                      of row i summed in the order of the other block index
 //                      (column-sum partials of blocks (L, K), L < K, then
@@ -47,7 +46,6 @@ template <class R, class S> struct SymArgs {
   const S *A;          // stored blocks
   const int2 *ij;      // (I, J) of every stored block
   long nb;             // blocks per row / column of the padded matrix
-  long nblocks;        // stored blocks, nb (nb + 1) / 2
   SymPair<Acc> *prow;  // [block][kSymT]: row sums (C, S) of the block
   SymPair<Acc> *pcol;  // [block][kSymT]: column sums of off-diagonal blocks
   R mscale;            // M = mscale * stored
@@ -126,12 +124,16 @@ template <class R> __device__ __forceinline__ R wave_reduce8(R (&v)[8], int lane
   return s;
 }
 
-// Shape of the block pass: wave w of the workgroup owns rows w, w + 4,
-// w + 8, ... of the block, RB at a time with the next RB in flight, so that
-// the workgroup's loads at any moment cover one contiguous stretch of the
-// block (6% faster than four separate quarter-blocks per workgroup); lane l
-// owns the columns q 64 VEC + l VEC + e (q < NLD, e < VEC): four per lane,
-// loaded 16 (8) bytes at a time, coalesced per row.
+// Shape of the block pass: wave w of the workgroup owns 64 rows of the
+// block, RB at a time with the next RB in flight; lane l owns the columns
+// q 64 VEC + l VEC + e (q < NLD, e < VEC): four per lane, loaded 16 (8)
+// bytes at a time, coalesced per row.  With 4- and 8-byte elements the
+// wave's rows are w, w + 4, w + 8, ..., so that the workgroup's loads at any
+// moment cover one contiguous stretch of the block; with 16-bit elements
+// (512-byte rows) they are the block's w-th quarter, four streams per
+// workgroup (each form measured faster for its element size: FP64 1.72 ms
+// against 1.76, FP32 0.88 against 0.90, BF16 0.47 against 0.50 per RHS at
+// N = 50000).
 template <class R, class S> struct SymPass {
   using Acc = typename Store<S>::Acc;
   using Val = typename Store<S>::Val;
@@ -139,12 +141,15 @@ template <class R, class S> struct SymPass {
   static constexpr int VEC = SymLoad<S>::VEC;
   static constexpr int NLD = kSymT / (kWave * VEC);
   static constexpr int NC = NLD * VEC;
-  static constexpr int RB = 4;
+  static constexpr int RB = 8;
   static constexpr int NW = kBlock / kWave;
   static constexpr int RW = kSymT / NW;
-  static_assert(NC == 4 && RW % (2 * RB) == 0, "block shape");
+  static constexpr bool QUARTER = sizeof(S) == 2;
+  static_assert(NC == 4 && RW % (2 * RB) == 0 && 2 * RB == 16, "block shape");
 
-  __device__ static int row_of(int wave, int k) { return wave + NW * k; }  // the wave's k-th row
+  // the wave's k-th row, and the row distance of consecutive k
+  static constexpr int KSTEP = QUARTER ? 1 : NW;
+  __device__ static int row_of(int wave, int k) { return (QUARTER ? wave * RW : wave) + KSTEP * k; }
 
   // The block as a buffer resource: its base in scalar registers, so a
   // lane's address is one 32-bit offset (voff, the lane's row-0 columns in
@@ -154,7 +159,7 @@ template <class R, class S> struct SymPass {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<S *>(blk_base), (short)0, (int)(kSymT * kSymT * sizeof(S)),
                                             0x00020000);
   }
-  __device__ static int lane_offset(int wave, int lane) { return (wave * kSymT + lane * VEC) * (int)sizeof(S); }
+  __device__ static int lane_offset(int wave, int lane) { return (row_of(wave, 0) * kSymT + lane * VEC) * (int)sizeof(S); }
 
   // rows kb .. kb + RB - 1 of the wave (lane's columns), non-temporal
   __device__ static void load(Raw (&raw)[RB][NLD], __amdgpu_buffer_rsrc_t rs, int voff, int kb) {
@@ -162,7 +167,7 @@ template <class R, class S> struct SymPass {
     for (int r = 0; r < RB; ++r)
 #pragma unroll
       for (int q = 0; q < NLD; ++q) {
-        const int soff = (NW * (kb + r) * kSymT + q * kWave * VEC) * (int)sizeof(S);
+        const int soff = (KSTEP * (kb + r) * kSymT + q * kWave * VEC) * (int)sizeof(S);
         if constexpr (sizeof(Raw) == 16) raw[r][q] = __builtin_amdgcn_raw_buffer_load_b128(rs, voff, soff, 2);
         else raw[r][q] = __builtin_amdgcn_raw_buffer_load_b64(rs, voff, soff, 2);
       }
@@ -199,10 +204,10 @@ template <class R, class S> struct SymPass {
       part[r] = pc;
       part[RB + r] = ps;
     }
-    // every lane stores its slot's sum (the 8 lanes of a slot hold the same
+    // every lane stores its slot's sum (the 4 lanes of a slot hold the same
     // value): no divergent branch around the store
-    const Acc tot = wave_reduce8<Acc>(part, lane);
-    const int slot = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
+    const Acc tot = wave_reduce16<Acc>(part, lane);
+    const int slot = ((lane >> 5) & 1) * 8 + ((lane >> 4) & 1) * 4 + ((lane >> 3) & 1) * 2 + ((lane >> 2) & 1);
     Acc *dst = reinterpret_cast<Acc *>(prow_blk + row_of(wave, kb + (slot % RB)));
     dst[slot / RB] = tot;
   }
@@ -304,80 +309,6 @@ __global__ __launch_bounds__(kBlock) void sym_tile_kernel(const SymArgs<R, S> a)
   P::process(raw1, RW - RB, wave, lane, vi, cj, sj, colC, colS, prow);
   if (I == J) return;  // the diagonal block is stored whole: its row sums are complete
   P::finish_columns(colbuf, colC, colS, wave, lane, tid, a.pcol + (size_t)blk * kSymT);
-}
-
-// The same block pass over a contiguous span of stored blocks per workgroup
-// (one resident round: workgroup g takes blocks [g B / G, (g + 1) B / G)),
-// pipelined across block boundaries: while a block's last two batches are
-// reduced, the next block's first two batches and its (cos, sin) vectors
-// are already loading, so a workgroup's stream of loads does not stop at
-// every block for the block's set-up and column reduction.  Every block is
-// summed exactly as sym_tile_kernel sums it: the partials are the same bits.
-template <class R, class S>
-__global__ __launch_bounds__(kBlock) void sym_span_kernel(const SymArgs<R, S> a) {
-  if (a.v.check_state && (!a.v.st->active || a.v.st->commit_pending)) return;
-  using P = SymPass<R, S>;
-  using Acc = typename P::Acc;
-  using Raw = typename P::Raw;
-  constexpr int RB = P::RB, NLD = P::NLD, NC = P::NC, NW = P::NW, RW = P::RW;
-  __shared__ Acc vi[2][kSymT];
-  __shared__ Acc colbuf[NW][2][kSymT];
-  const int tid = threadIdx.x, lane = tid % kWave, wave = tid / kWave;
-  const long nblk = a.nblocks;
-  const long b0 = (long)blockIdx.x * nblk / gridDim.x, b1 = (long)(blockIdx.x + 1) * nblk / gridDim.x;
-  if (b0 >= b1) return;
-  const long N = a.v.N;
-  const R *cosv = a.v.cs_in, *sinv = a.v.cs_in + a.v.ldv;
-  const int voff = P::lane_offset(wave, lane);
-  __amdgpu_buffer_rsrc_t rs = P::block_rsrc(a.A + (size_t)b0 * kSymT * kSymT);
-  Raw raw0[RB][NLD], raw1[RB][NLD];
-  P::load(raw0, rs, voff, 0);
-  P::load(raw1, rs, voff, RB);
-  int2 ij = a.ij[b0];
-  Acc vic, vis, cj[NC], sj[NC];
-  P::vec_at(cosv, sinv, (long)ij.x * kSymT + tid, N, vic, vis);
-  P::col_vectors(cosv, sinv, ij.y, N, lane, cj, sj);
-#pragma unroll 1
-  for (long blk = b0; blk < b1; ++blk) {
-    // the next block, or (after the span's last block) a resource of no
-    // bytes: its loads return zeros without touching memory
-    const bool next = blk + 1 < b1;
-    const long nb = next ? blk + 1 : blk;
-    const int2 ij_next = a.ij[nb];
-    const __amdgpu_buffer_rsrc_t rsn = __builtin_amdgcn_make_buffer_rsrc(
-        const_cast<S *>(a.A + (size_t)nb * kSymT * kSymT), (short)0,
-        next ? (int)(kSymT * kSymT * sizeof(S)) : 0, 0x00020000);
-    const long I = ij.x, J = ij.y;
-    if (blk > b0) __syncthreads();  // the previous block's reads of vi and colbuf are done
-    vi[0][tid] = vic;
-    vi[1][tid] = vis;
-    __syncthreads();
-    Acc colC[NC], colS[NC];
-#pragma unroll
-    for (int k = 0; k < NC; ++k) { colC[k] = 0; colS[k] = 0; }
-    SymPair<Acc> *prow = a.prow + (size_t)blk * kSymT;
-#pragma unroll 1
-    for (int kb = 0; kb < RW - 2 * RB; kb += 2 * RB) {
-      P::process(raw0, kb, wave, lane, vi, cj, sj, colC, colS, prow);
-      P::load(raw0, rs, voff, kb + 2 * RB);
-      P::process(raw1, kb + RB, wave, lane, vi, cj, sj, colC, colS, prow);
-      P::load(raw1, rs, voff, kb + 3 * RB);
-    }
-    // the last pair; the next block's vectors and first two batches load
-    Acc ncj[NC], nsj[NC];
-    P::process(raw0, RW - 2 * RB, wave, lane, vi, cj, sj, colC, colS, prow);
-    P::vec_at(cosv, sinv, (long)ij_next.x * kSymT + tid, N, vic, vis);
-    P::col_vectors(cosv, sinv, ij_next.y, N, lane, ncj, nsj);
-    P::load(raw0, rsn, voff, 0);
-    P::process(raw1, RW - RB, wave, lane, vi, cj, sj, colC, colS, prow);
-    P::load(raw1, rsn, voff, RB);
-    if (I != J)  // the diagonal block is stored whole: its row sums are complete
-      P::finish_columns(colbuf, colC, colS, wave, lane, tid, a.pcol + (size_t)blk * kSymT);
-#pragma unroll
-    for (int k = 0; k < NC; ++k) { cj[k] = ncj[k]; sj[k] = nsj[k]; }
-    ij = ij_next;
-    rs = rsn;
-  }
 }
 
 // One thread per row; a workgroup covers exactly one block row (kBlock =

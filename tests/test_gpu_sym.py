@@ -243,31 +243,3 @@ def test_sym_bf16_fp64_integration_follows_dense_bf16(km):
     assert np.max(np.abs(a["r"] - b["r"])) < 1e-5
     assert abs(a["status"].accepted - b["status"].accepted) <= max(1, int(0.02 * a["status"].accepted))
 
-
-@pytest.mark.parametrize("N", [600, 12345])
-@pytest.mark.parametrize("prec,storage", [("f32", "sym"), ("f64", "sym"), ("f64", "sym-bf16"), ("f32", "sym-f16")])
-def test_sym_span_equals_tile(km, prec, storage, N):
-    """The block pass over contiguous spans of blocks per resident workgroup
-    (select_variant balanced=True) sums every block exactly as the one-block-
-    per-workgroup pass: RHS, couplings and a whole integration bit for bit.
-    N = 12345 (1225 blocks, the last block row ragged) gives spans of two
-    or more blocks per workgroup; N = 600 (6 blocks) one block each."""
-    p = km.F32 if prec == "f32" else km.F64
-    dtype, theta0, omega, X, w = _factor_case(km, p, N, 11)
-    te = _grid(0.2, 0.05, dtype)
-    print(f"case sym span N={N} {prec} {storage}", flush=True)
-    out = {}
-    for span in (False, True):
-        with km.Integrator(N, p, atol=1e-6 if prec == "f32" else 1e-9, c1=3.0) as it:
-            it.set_omega(omega)
-            it.set_factor(X, w)
-            it.build_adjacency(storage)
-            it.select_variant(8, True, span)
-            f, cp = it.rhs(theta0, couplings=True)
-            it.set_state(theta0, 0.0, 0.01)
-            g = it.run(te, 0.2, keep_rows=True, ring_rows=8)
-            out[span] = (f, cp, g)
-    (f0, c0, g0), (f1, c1, g1) = out[False], out[True]
-    assert np.array_equal(f0, f1) and np.array_equal(c0, c1)
-    assert np.array_equal(g0["rows"], g1["rows"]) and np.array_equal(g0["times"], g1["times"])
-    assert g0["status"].accepted == g1["status"].accepted
