@@ -168,7 +168,9 @@ __global__ __launch_bounds__(WAVES * kWave) void bgemm_kernel(const T *At, long 
   constexpr int VPIECES = MT * 32 * 8;  // 16-B pieces of one V^T chunk (hi or lo)
   constexpr int VPT = (VPIECES + NTHR - 1) / NTHR;
   __shared__ uint16_t sh[2][NV][MT * 32 * kLdsRow];
-  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  // the wave index in a scalar register: the band's buffer resource below
+  // must be wave-uniform
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = threadIdx.x % kWave;
   const int slab = (int)(blockIdx.x % ksplit);
   const long rb = blockIdx.x / ksplit;
   const long band0 = (rb * WAVES + wave) * NT;
@@ -188,23 +190,41 @@ __global__ __launch_bounds__(WAVES * kWave) void bgemm_kernel(const T *At, long 
 
   u32x4 a[PF + 1][NT][4];
   u32x4 vr[VD][NV][VPT];
-  auto load_a = [&](long c, u32x4 (&dst)[NT][4]) {
+  // Every load and LDS store in the chunk loop is unconditional: a load
+  // behind a branch makes the compiler wait, where the paths join, for every
+  // load in flight, and the A prefetch would drain once per loop period.
+  // Band nt of A over this slab's chunks is a buffer resource of exactly
+  // those chunks: prefetches past the slab's last chunk return zeros
+  // without memory traffic (the chunk offset is in the lane offset, which
+  // the hardware range-checks).
+  __amdgpu_buffer_rsrc_t ars[NT];
 #pragma unroll
-    for (int nt = 0; nt < NT; ++nt) {
-      const T *tile = At + ((size_t)(band0 + nt) * nkc + c) * kTile + lane * 8;
+  for (int nt = 0; nt < NT; ++nt)
+    ars[nt] = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<T *>(At + ((size_t)(band0 + nt) * nkc + c0) * kTile), (short)0,
+        (int)(c1 > c0 ? (c1 - c0) * kTile * (long)sizeof(T) : 0), 0x00020000);
+  auto load_a = [&](long c, u32x4 (&dst)[NT][4]) {
+    const int base = (int)((c - c0) * kTile * (long)sizeof(T)) + lane * 16;
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt)
 #pragma unroll
       for (int ks = 0; ks < 4; ++ks)
-        dst[nt][ks] = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(tile + ks * 512));
-    }
+        dst[nt][ks] = __builtin_amdgcn_raw_buffer_load_b128(ars[nt], base + ks * 1024, 0, 2);
+  };
+  // V^T pieces: past the last piece a thread repeats the last one (the same
+  // bytes to the same LDS place), past the slab the last chunk
+  auto vpiece = [&](int u) {
+    const int p = threadIdx.x + u * NTHR;
+    return VPIECES % NTHR == 0 || p < VPIECES ? p : VPIECES - 1;
   };
   auto load_v = [&](long c, u32x4 (&vr)[NV][VPT]) {
+    const long cc = c < c1 ? c : c1 - 1;
 #pragma unroll
     for (int q = 0; q < NV; ++q)
 #pragma unroll
       for (int u = 0; u < VPT; ++u) {
-        const int p = threadIdx.x + u * NTHR;
-        if (VPIECES % NTHR == 0 || p < VPIECES)
-          vr[q][u] = *reinterpret_cast<const u32x4 *>(vsrc[q] + (size_t)(p >> 3) * ldk + c * kKC + (p & 7) * 8);
+        const int p = vpiece(u);
+        vr[q][u] = *reinterpret_cast<const u32x4 *>(vsrc[q] + (size_t)(p >> 3) * ldk + cc * kKC + (p & 7) * 8);
       }
   };
   auto store_v = [&](const u32x4 (&vr)[NV][VPT], int buf) {
@@ -212,9 +232,8 @@ __global__ __launch_bounds__(WAVES * kWave) void bgemm_kernel(const T *At, long 
     for (int q = 0; q < NV; ++q)
 #pragma unroll
       for (int u = 0; u < VPT; ++u) {
-        const int p = threadIdx.x + u * NTHR;
-        if (VPIECES % NTHR == 0 || p < VPIECES)
-          *reinterpret_cast<u32x4 *>(&sh[buf][q][(p >> 3) * kLdsRow + (p & 7) * 8]) = vr[q][u];
+        const int p = vpiece(u);
+        *reinterpret_cast<u32x4 *>(&sh[buf][q][(p >> 3) * kLdsRow + (p & 7) * 8]) = vr[q][u];
       }
   };
   // The MFMAs of one chunk: 4 k-steps x (hi, lo) x MT tiles of V^T from
@@ -281,10 +300,15 @@ __global__ __launch_bounds__(WAVES * kWave) void bgemm_kernel(const T *At, long 
     constexpr int NX = (S + PF) % (PF + 1);
     constexpr int SV = I % VD;
     const int cur = (int)((c - c0) & 1);
-    if (c + VD < c1) load_v(c + VD, vr[SV]);
-    if (c + PF < c1) load_a(c + PF, a[NX]);
+    // the order is fixed: a load issued after the LDS store would make the
+    // store's wait cover the A chunk that is still a step ahead
+    __builtin_amdgcn_sched_barrier(0);
+    load_v(c + VD, vr[SV]);
+    load_a(c + PF, a[NX]);
+    __builtin_amdgcn_sched_barrier(0);
     multiply(a[S], cur);
-    if (c + 1 < c1) store_v(vr[(SV + 1) % VD], cur ^ 1);
+    __builtin_amdgcn_sched_barrier(0);
+    store_v(vr[(SV + 1) % VD], cur ^ 1);  // (after the last chunk: into the buffer no one reads again)
     __syncthreads();
   };
 
