@@ -355,8 +355,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="fp64-n50000", choices=sorted(CONFIGS) + sorted(BATCH_CONFIGS))
-    ap.add_argument("--cpu-steps", type=int, default=50,
-                    help="accepted steps of the CPU baseline sample: a fixed window from t=0 (0: skip)")
+    ap.add_argument("--cpu-steps", type=int, default=30,
+                    help="accepted steps of the CPU baseline sample: a fixed window from t=0 (0: skip); "
+                         "30 steps of the FP64 N=50000 default take 15-25 s on 16 host threads")
     ap.add_argument("--batch-t-end", type=float, default=10.0,
                     help="batched workloads: end time of the whole runs that compare the step controllers "
                          "(and, for the shared batch, 64 sequential plain runs); 0 skips them")
