@@ -226,10 +226,12 @@ def batch_main(args):
     secs = r["region_ms"] / 1e3
     osc_steps = st.osc_steps - st0.osc_steps
     Ns = r["Ns"]
+    hilo = cfg.get("hilo", True)
+    storage = cfg.get("storage", "bf16")
     if cfg["shared"]:
         N = Ns[0]
         mpad = 128
-        alg = 2 * N * N + 2 * 2 * mpad * N  # BF16 A once + hi/lo V^T once
+        alg = 2 * N * N + (2 if hilo else 1) * 2 * mpad * N  # 16-bit A once + the hi (and lo) V^T once
         seq_ms = plain_attempt_ms(N)
         inst_steps = st.accepted - st0.accepted
         per_attempt_batched = r["region_ms"] / args.steps
@@ -284,13 +286,15 @@ def batch_main(args):
         "unit": "osc*steps/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": r["region_ms"] / args.steps,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 (bf16 adjacency)",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": f"f32 ({storage} adjacency" + ("" if hilo else f", {storage} cos/sin in the product") + ")",
         "data": "synthetic (Philox Gaussian factors, glibc-rand frequencies/phases)",
         "config": {"workload": args.config, "B": cfg["B"], "instance_steps_in_region": st.accepted - st0.accepted,
                    **extra},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": profiled_traffic(args.config, kname),
-                     "kernel": kname + (" (MFMA, hi+lo vectors)" if cfg["shared"] else " (grouped gemv-pairs)"),
+                     "kernel": kname + ((" (MFMA, hi+lo vectors)" if hilo else " (MFMA, hi vectors only)")
+                                        if cfg["shared"] else " (grouped gemv-pairs)"),
                      "bytes_per_launch": alg, "avg_launch_ms": r["coupling_ms"],
                      "measured_read_stream_gbps": read_gbps, "measured_copy_gbps": copy_gbps},
     }

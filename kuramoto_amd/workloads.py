@@ -62,6 +62,12 @@ BATCH = {
     # Cauchy frequencies, theta0 = pi(2u-1), seeds 1..64, t=50
     "batch64-n20000-bf16": dict(B=64, Ns=[20000] * 64, shared=True, seeds=list(range(1, 65)),
                                 cs=list(np.linspace(0.5, 4.0, 64)), normal=[False] * 64, t1=50.0, dt=1.0),
+    # the same sweep with the adjacency in FP16 and the cos/sin vectors of the
+    # product in FP16 only (11 significant bits, one MFMA per tile instead of
+    # the hi + lo pair that keeps them FP32-class)
+    "batch64-n20000-f16-hi": dict(B=64, Ns=[20000] * 64, shared=True, seeds=list(range(1, 65)),
+                                  cs=list(np.linspace(0.5, 4.0, 64)), normal=[False] * 64, t1=50.0, dt=1.0,
+                                  storage="f16", hilo=False),
     # B=48, N uniform in {5000..40000}, own adjacency each (seeds
     # 100..147), normal/Cauchy alternating, c=3, t=100, with the learned
     # step proposer against the same batch without it
@@ -83,7 +89,8 @@ def make_batch(name, proposer, tile=None):
     cfg = BATCH[name]
     B = cfg["B"]
     Ns = cfg["Ns"] or uneven_sizes(B)
-    bt = km.Batch(Ns, atol=1e-6, shared=cfg["shared"], storage="bf16")
+    bt = km.Batch(Ns, atol=1e-6, shared=cfg["shared"], storage=cfg.get("storage", "bf16"))
+    bt.set_vector_split(cfg.get("hilo", True))
     if cfg["shared"]:
         X = km.factor(Ns[0], 5, 1, gaussian=True).astype(np.float32)
         bt.set_factor(-1, X, km.factor_weights(km.F32, Ns[0], 5, gaussian=True))

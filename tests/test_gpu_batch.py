@@ -126,6 +126,39 @@ def test_shared_autotune_and_split(km):
         assert np.max(np.abs(out[True]["r"][b] - out[False]["r"][b])) < 1e-3
 
 
+def test_shared_f16_hi_only_matches_fp64_oracle(km, orc):
+    """FP16 adjacency with FP16 cos/sin only in the product (one MFMA per
+    tile, the batch64-n20000-f16-hi bench form): r(t) of every checked
+    instance within 1e-3 of the FP64 oracle on the same stored matrix, and
+    accepted steps within 2% (or 1) of the hi + lo run of the same batch."""
+    B, N = 16, 2048
+    X = km.factor(N, 5, 12, gaussian=True).astype(np.float32)
+    w = km.factor_weights(km.F32, N, 5, gaussian=True)
+    cs = np.linspace(0.5, 4.0, B)
+    t1, te = 1.0, _grid(1.0, 0.25)
+    draws = [km.draws(km.F32, N, 200 + b, normal=True, init=6.283185307) for b in range(B)]
+    print("case shared f16 hi-only", flush=True)
+    out = {}
+    for split in (True, False):
+        with km.Batch([N] * B, atol=1e-6, shared=True, storage="f16") as bt:
+            for b in range(B):
+                bt.set_instance(b, draws[b][1], draws[b][0], cs[b], 0.0, 0.01)
+            bt.set_factor(-1, X, w)
+            bt.set_vector_split(split)
+            Mq = bt.adjacency_rows(-1, 0, N)
+            res = bt.run(te, t1, ring_rows=64, chunk=8)
+            out[split] = (res, [bt.get_instance(b)[1] for b in range(B)], Mq)
+    res, st, Mq = out[False]
+    assert res["status"].active == 0 and res["status"].errors == 0
+    for b in range(B):
+        a_hl, a_hi = out[True][1][b]["accepted"], st[b]["accepted"]
+        assert abs(a_hi - a_hl) <= max(1, int(0.02 * a_hl)), b
+    for b in (0, 7, 15):
+        ref = orc.integrate(orc.Adj(0, M=Mq.astype(np.float64)), draws[b][1].astype(np.float64), cs[b],
+                            draws[b][0].astype(np.float64), 0.0, 0.01, t1, 1e-9, t_eval=te, keep_rows=False)
+        assert np.max(np.abs(res["r"][b] - ref["r"])) < 1e-3, b
+
+
 def test_attempt_log_and_proposer_fallback(km):
     """The attempt log records every attempt; a proposer (here: weights that
     propose 2x the standard step) is used after accepts, the embedded error
