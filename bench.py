@@ -98,10 +98,11 @@ def gpu_run(cfg, steps, warmup):
     # add a few microseconds between launches, so they stay out of it)
     _, stage_ms, _ = it.time_attempts(min(steps, 10))
     lda, eb, rows, nt, bal = it.layout()
+    ring = bool(it.variant_flags() & 4)
     it.close()
     copy_gbps, read_gbps = km.bandwidth_probe(8 << 30, 5)
     return dict(dtype=dtype, N=N, region_ms=region_ms, stage_ms=stage_ms, st0=st0, st=st, lda=lda, elem_bytes=eb,
-                gen_s=gen_s, rows=rows, nt=nt, balanced=bal, copy_gbps=copy_gbps, read_gbps=read_gbps)
+                gen_s=gen_s, rows=rows, nt=nt, balanced=bal, ring=ring, copy_gbps=copy_gbps, read_gbps=read_gbps)
 
 
 def cpu_baseline(cfg, max_steps):
@@ -449,7 +450,7 @@ def main():
                      "traffic": profiled_traffic(args.config, "lowrank_stage_kernel" if cfg["storage"] == "factored"
                                                  else "sym_tile_kernel" if cfg["storage"].startswith("sym")
                                                  else "sign_mfma_kernel" if cfg["storage"] == "signs"
-                                                 else "stage_kernel"),
+                                                 else "stage_ring_kernel" if r["ring"] else "stage_kernel"),
                      "kernel": ("sign_mfma_kernel x 2 (int8 MFMA passes over the bits) + sign_limbs / sign_gram / "
                                 "sign_stage kernels (one RHS)"
                                 if cfg["storage"] == "signs" else
@@ -457,11 +458,12 @@ def main():
                                 else "rebuild_stage_kernel (tiles rebuilt in registers)" if cfg["storage"] == "rebuild"
                                 else "sym_tile_kernel + sym_reduce_kernel (one RHS from the upper block triangle)"
                                 if cfg["storage"].startswith("sym")
-                                else "stage_kernel (fused gemv-pair + DP45 stage epilogue)"),
+                                else "stage_ring_kernel (fused gemv-pair + DP45 stage epilogue, chunk ring)"
+                                if r["ring"] else "stage_kernel (fused gemv-pair + DP45 stage epilogue)"),
                      "bytes_per_launch": alg_bytes, "avg_launch_ms": r["stage_ms"],
                      "variant": ({"block_edge": r["lda"]} if cfg["storage"].startswith("sym") else
                                  {"rows_per_group": r["rows"], "nontemporal_loads": r["nt"],
-                                  "balanced_grid": r["balanced"]}),
+                                  "balanced_grid": r["balanced"], "chunk_ring": r["ring"]}),
                      "measured_read_stream_gbps": r["read_gbps"], "measured_copy_gbps": r["copy_gbps"],
                      "frac_of_measured_read_stream": achieved / 1e9 / r["read_gbps"]},
     }

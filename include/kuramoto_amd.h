@@ -247,7 +247,8 @@ int kmt_select_variant(kmt_ctx *ctx, int rows_per_block, int variant_flags);
  * the adjacency in elements (KMT_ADJ_SYM: the block edge, 256), bytes per
  * stored element, rows per group of the
  * stage kernel, and variant_flags (bit 0: non-temporal adjacency loads, bit
- * 1: balanced grid).  Any pointer may be NULL. */
+ * 1: balanced grid, bit 2: chunk ring — each row's next chunk loads while
+ * the current one is reduced; not with bit 1).  Any pointer may be NULL. */
 int kmt_layout(kmt_ctx *ctx, long *lda, int *elem_bytes, int *rows_per_block, int *variant_flags);
 
 /* ---- batched integration (an extension; no reference function:

@@ -377,9 +377,16 @@ class Integrator:
         _check(self.lib.kmt_autotune(self.ctx, reps, _ptr(ms)), "kmt_autotune")
         return ms.reshape(3, 2, 2)
 
-    def select_variant(self, rows, nontemporal=True, balanced=False):
-        _check(self.lib.kmt_select_variant(self.ctx, rows, int(nontemporal) | (2 if balanced else 0)),
-               "kmt_select_variant")
+    def select_variant(self, rows, nontemporal=True, balanced=False, ring=False):
+        _check(self.lib.kmt_select_variant(self.ctx, rows, int(nontemporal) | (2 if balanced else 0)
+                                           | (4 if ring else 0)), "kmt_select_variant")
+
+    def variant_flags(self):
+        """kmt_layout's variant_flags: bit 0 non-temporal loads, bit 1
+        balanced grid, bit 2 chunk ring."""
+        fl = ctypes.c_int()
+        _check(self.lib.kmt_layout(self.ctx, None, None, None, ctypes.byref(fl)), "kmt_layout")
+        return fl.value
 
     def layout(self):
         """(lda, bytes per stored element, rows per group, non-temporal, balanced)."""

@@ -116,7 +116,7 @@ template <class R> struct Engine : EngineBase {
   int ring_cap = 0, keep_rows = 0, times_cap = 4096, nblk_emit = 0;
   hipGraphExec_t g1 = nullptr, g8 = nullptr;
   bool graphs_valid = false;
-  int tune_rows = 8, tune_nt = 1, tune_bal = 0, tuned = 0;
+  int tune_rows = 8, tune_nt = 1, tune_bal = 0, tune_ring = 0, tuned = 0;
   double tune_ms[12] = {0};
   int bal_grid = 0;  // workgroups of the balanced variant (one resident round)
   R t0_ = 0, h0_ = 0;
@@ -206,7 +206,7 @@ template <class R> struct Engine : EngineBase {
     if (plda) *plda = sym() ? kSymT : lda;
     if (eb) *eb = storage < 0 ? (int)sizeof(R) : (int)elem_bytes(storage);
     if (rows) *rows = tune_rows;
-    if (flags) *flags = (tune_nt ? 1 : 0) | (tune_bal ? 2 : 0);
+    if (flags) *flags = (tune_nt ? 1 : 0) | (tune_bal ? 2 : 0) | (tune_ring ? 4 : 0);
     return KMT_OK;
   }
   // The stage-7 error partials are summed per workgroup, so their grouping,
@@ -221,12 +221,16 @@ template <class R> struct Engine : EngineBase {
   // rows: each (cos, sin) chunk is reused for every row of the group from
   // registers, and at 8 rows the vectors' L2 bytes would match the 2-byte
   // entries' (FP64 state, BF16, N=50000: 0.754 against 0.811 ms per stage).
+  // A matrix in the working precision that streams from HBM takes the chunk
+  // ring (the same sums and grouping, 0.6% faster at N=50000 FP64 and FP32;
+  // slower with 16-bit entries at 16 rows, where it needs 231 registers).
   // kmt_autotune / kmt_select_variant override it.
   void default_variant() {
     const bool cached = N >= 4096 && (double)N * (double)N * (double)elem_bytes(storage) <= 128e6;
     tune_rows = cached ? 4 : (elem_bytes(storage) == 2 ? 16 : 8);
     tune_nt = cached ? 0 : 1;
     tune_bal = 0;
+    tune_ring = !cached && elem_bytes(storage) != 2;
     bal_grid = 0;
     tuned = 1;
     drop_graphs();
@@ -234,10 +238,12 @@ template <class R> struct Engine : EngineBase {
 
   int select_variant(int rows, int flags) override {
     if (rows != 4 && rows != 8 && rows != 16) return fail(KMT_ERR_ARG, "rows per group must be 4, 8 or 16 (got %d)", rows);
-    if (flags & ~3) return fail(KMT_ERR_ARG, "unknown variant flags 0x%x", flags);
+    if (flags & ~7) return fail(KMT_ERR_ARG, "unknown variant flags 0x%x", flags);
+    if ((flags & 6) == 6) return fail(KMT_ERR_ARG, "the chunk ring has one workgroup per row group (not balanced)");
     tune_rows = rows;
     tune_nt = flags & 1;
     tune_bal = (flags >> 1) & 1;
+    tune_ring = (flags >> 2) & 1;
     bal_grid = 0;
     tuned = 1;
     drop_graphs();
@@ -768,6 +774,9 @@ template <class R> struct Engine : EngineBase {
     if (tune_bal) {
       if (bal_grid <= 0) bal_grid = balanced_grid<S, ROWS, NT>();
       stage_kernel<R, S, ROWS, KIND, NT, true><<<(unsigned)bal_grid, kBlock, 0, stream>>>(a);
+    } else if (tune_ring) {
+      const unsigned grid = (unsigned)((N + ROWS - 1) / ROWS);
+      stage_ring_kernel<R, S, ROWS, KIND, NT><<<grid, kBlock, 0, stream>>>(a);
     } else {
       const unsigned grid = (unsigned)((N + ROWS - 1) / ROWS);
       stage_kernel<R, S, ROWS, KIND, NT, false><<<grid, kBlock, 0, stream>>>(a);
@@ -976,6 +985,7 @@ template <class R> struct Engine : EngineBase {
         tune_rows = rows_opts[ri];
         tune_nt = nt;
         tune_bal = bal;
+        tune_ring = 0;
         bal_grid = 0;
         launch_by_kind(3, v);
         HIP_TRY(hipEventRecord(e0, stream));

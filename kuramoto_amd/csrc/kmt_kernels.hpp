@@ -428,6 +428,98 @@ __global__ __launch_bounds__(kBlock) void stage_kernel(const DenseArgs<R, S> a) 
   }
 }
 
+// The same group pass with the adjacency streamed through a ring of two
+// chunk buffers: chunk k + 1 of every row loads while chunk k is reduced,
+// so each wave always has one to two chunks of loads in flight (the loop
+// above issues a pair of chunks and then waits for all of it) with the
+// same two register sets.  Rows and vectors are buffer resources of lda
+// elements: the lanes of the last, partial chunk read zeros (no memory
+// access) instead of branching, every load in the loop is unconditional,
+// and each chunk's loads are issued before the previous chunk's products.
+// Every row is summed in the same order as stage_kernel: chunk by chunk, VEC
+// values each; a zero chunk adds +0 to sums that are never -0, so the
+// results are the same bits.  One group per workgroup.
+template <class R, class S, int ROWS, int KIND, bool NT>
+__global__ __launch_bounds__(kBlock) void stage_ring_kernel(const DenseArgs<R, S> a) {
+  if constexpr (KIND >= 2) {
+    if (a.v.check_state && (!a.v.st->active || a.v.st->commit_pending)) return;
+  }
+  using Acc = typename Store<S>::Acc;
+  using Val = typename Store<S>::Val;
+  constexpr int VEC = Store<S>::VEC;
+  constexpr long STRIDE = (long)kBlock * VEC;
+  constexpr int WV = (int)(VEC * sizeof(R) / 16);  // 16-byte loads per vector chunk
+  const int tid = threadIdx.x;
+  const long N = a.v.N;
+  const long row0 = (long)blockIdx.x * ROWS;
+  const long nch = (a.lda + STRIDE - 1) / STRIDE;
+  __amdgpu_buffer_rsrc_t rrow[ROWS];
+#pragma unroll
+  for (int r = 0; r < ROWS; ++r)
+    rrow[r] = __builtin_amdgcn_make_buffer_rsrc(const_cast<S *>(a.A + (row0 + r) * a.lda), (short)0,
+                                                (int)(a.lda * (long)sizeof(S)), 0x00020000);
+  const __amdgpu_buffer_rsrc_t rcos = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<R *>(a.v.cs_in), (short)0, (int)(a.lda * (long)sizeof(R)), 0x00020000);
+  const __amdgpu_buffer_rsrc_t rsin = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<R *>(a.v.cs_in + a.v.ldv), (short)0, (int)(a.lda * (long)sizeof(R)), 0x00020000);
+  constexpr int AUX = NT ? 2 : 0;
+  struct Chunk {
+    u32x4 raw[ROWS];
+    u32x4 cv[WV], sv[WV];
+  };
+  auto load = [&](long k, Chunk &ch) {
+    const long c = k * STRIDE + (long)tid * VEC;
+    const int vo = (int)(c * (long)sizeof(R));
+#pragma unroll
+    for (int w = 0; w < WV; ++w) {
+      ch.cv[w] = __builtin_amdgcn_raw_buffer_load_b128(rcos, vo + 16 * w, 0, 0);
+      ch.sv[w] = __builtin_amdgcn_raw_buffer_load_b128(rsin, vo + 16 * w, 0, 0);
+    }
+    const int mo = (int)(c * (long)sizeof(S));
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) ch.raw[r] = __builtin_amdgcn_raw_buffer_load_b128(rrow[r], mo, 0, AUX);
+  };
+  Acc accC[ROWS], accS[ROWS];
+#pragma unroll
+  for (int r = 0; r < ROWS; ++r) { accC[r] = 0; accS[r] = 0; }
+  auto reduce = [&](const Chunk &ch) {
+    R cv[VEC], sv[VEC];
+#pragma unroll
+    for (int w = 0; w < WV; ++w) {
+      unpack<R>(ch.cv[w], cv + (16 / sizeof(R)) * w);
+      unpack<R>(ch.sv[w], sv + (16 / sizeof(R)) * w);
+    }
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+      Val m[VEC];
+      unpack<S>(ch.raw[r], m);
+#pragma unroll
+      for (int q = 0; q < VEC; ++q) {
+        accC[r] = rfma<Acc>((Acc)m[q], (Acc)cv[q], accC[r]);
+        accS[r] = rfma<Acc>((Acc)m[q], (Acc)sv[q], accS[r]);
+      }
+    }
+  };
+  Chunk c0, c1;
+  load(0, c0);
+  // chunk pairs; the loads past the last chunk return zeros without memory
+  // traffic, so the loop needs no condition around them
+  long k = 0;
+#pragma unroll 1
+  for (; k + 1 < nch; k += 2) {
+    __builtin_amdgcn_sched_barrier(0);
+    load(k + 1, c1);
+    __builtin_amdgcn_sched_barrier(0);
+    reduce(c0);
+    __builtin_amdgcn_sched_barrier(0);
+    load(k + 2, c0);
+    __builtin_amdgcn_sched_barrier(0);
+    reduce(c1);
+  }
+  reduce(c0);  // chunk nch - 1 for an odd count, zeros otherwise
+  finish_rows<R, Acc, ROWS, KIND>(accC, accS, row0, a.mscale, a.v);
+}
+
 // ---------------------------------------------------------------------
 // -A with register-rebuilt adjacency tiles: M_ij = sum_m w_m (X_im X_jm)
 // is formed in registers for every (row, column) pair of a stage from the

@@ -209,3 +209,38 @@ def test_stage_variants_agree(km, orc, prec, N):
     a, b = runs[False], runs[True]
     assert abs(a["status"].accepted - b["status"].accepted) <= 1
     assert np.max(np.abs(a["rows"][-1] - b["rows"][-1])) < (1e-4 if prec == "f32" else 1e-10)
+
+
+@pytest.mark.parametrize("prec,storage", [("f32", "native"), ("f64", "native"), ("f32", "bf16"), ("f64", "bf16"),
+                                          ("f32", "f16")])
+@pytest.mark.parametrize("N", [700, 3333, 12345])
+def test_stage_ring_is_bit_identical(km, prec, storage, N):
+    """The chunk-ring stage kernel (variant flag 4: each row's next chunk
+    loads while the current one is reduced; the last, partial chunk read as
+    zeros through buffer bounds) sums every row in the stage kernel's order
+    and groups the error partials the same way: RHS, couplings and a whole
+    integration are the same bits as the plain variant of the same rows per
+    group.  N = 700, 3333, 12345 leave a partial last chunk and a partial
+    last row group."""
+    p = km.F32 if prec == "f32" else km.F64
+    dtype = np.float32 if prec == "f32" else np.float64
+    X = km.factor(N, 5, 23, gaussian=True).astype(dtype)
+    w = km.factor_weights(p, N, 5, gaussian=True)
+    th, om = km.draws(p, N, 23, normal=True, init=6.283185307)
+    te = np.arange(1, 5, dtype=np.float64) * 0.25
+    print(f"case ring N={N} prec={prec} storage={storage}", flush=True)
+    with km.Integrator(N, p, atol=1e-6 if prec == "f32" else 1e-9, c1=3.0) as it:
+        it.set_omega(om)
+        it.set_factor(X, w)
+        it.build_adjacency(storage)
+        for rows in (4, 8, 16):
+            out = {}
+            for ring in (False, True):
+                it.select_variant(rows, True, False, ring)
+                f, cp = it.rhs(th, couplings=True)
+                it.set_state(th, 0.0, 0.01)
+                g = it.run(te, 1.0, keep_rows=True)
+                out[ring] = (f, cp, g)
+            (f0, c0, g0), (f1, c1, g1) = out[False], out[True]
+            assert np.array_equal(f0, f1) and np.array_equal(c0, c1), rows
+            assert np.array_equal(g0["rows"], g1["rows"]) and np.array_equal(g0["times"], g1["times"]), rows
