@@ -100,30 +100,6 @@ template <class R> __device__ __forceinline__ R wave_reduce16(R (&v)[16], int la
   return s;
 }
 
-// The same for 8 values: lane l returns the sum of slot 4 b5 + 2 b4 + b3.
-template <class R> __device__ __forceinline__ R wave_reduce8(R (&v)[8], int lane) {
-  const bool b5 = lane & 32, b4 = lane & 16, b3 = lane & 8;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const R keep = b5 ? v[k + 4] : v[k], send = b5 ? v[k] : v[k + 4];
-    v[k] = keep + __shfl_xor(send, 32);
-  }
-#pragma unroll
-  for (int k = 0; k < 2; ++k) {
-    const R keep = b4 ? v[k + 2] : v[k], send = b4 ? v[k] : v[k + 2];
-    v[k] = keep + __shfl_xor(send, 16);
-  }
-  R s;
-  {
-    const R keep = b3 ? v[1] : v[0], send = b3 ? v[0] : v[1];
-    s = keep + __shfl_xor(send, 8);
-  }
-  s += __shfl_xor(s, 4);
-  s += __shfl_xor(s, 2);
-  s += __shfl_xor(s, 1);
-  return s;
-}
-
 // Shape of the block pass: wave w of the workgroup owns 64 rows of the
 // block, RB at a time with the next RB in flight; lane l owns the columns
 // q 64 VEC + l VEC + e (q < NLD, e < VEC): four per lane, loaded 16 (8)
