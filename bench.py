@@ -19,6 +19,10 @@ Printed JSON line (one):
                 library's stream), against 8 TB/s
   cpu_baseline = the oracle (FP64 C++ transcription, OpenMP) on the same
                 workload, first accepted steps only (bounded sample)
+  secondary   = (default workload only) the FP32 N=50000 dense run and the
+                64-instance shared BF16 batch, measured in the same process
+                after the headline figure (per-RHS roofline; batched vs 64
+                plain attempts)
 """
 import argparse
 import json
@@ -354,6 +358,37 @@ def published_main(args):
                       "runs": rows}), flush=True)
 
 
+def secondary_measurements(steps, warmup):
+    """Two more workloads measured in the same process after the headline
+    one, so that one bench run also covers them: the FP32 N=50000 dense run
+    (the plain integrator's target configuration, >= 85% of 8 TB/s per RHS)
+    and the 64-instance shared-adjacency batch (BF16 A, MFMA product) per
+    attempt against 64 plain FP32 attempts of the same size.  No whole runs
+    to a fixed time here (those take minutes; --config batch64-n20000-bf16
+    reports them)."""
+    out = {}
+    cfg = CONFIGS["fp32-n50000"]
+    r = gpu_run(cfg, steps, warmup)
+    N = r["N"]
+    alg = r["elem_bytes"] * N * N + N * 3 * 4
+    out["fp32-n50000"] = {
+        "value_osc_steps_per_s": N * (r["st"].accepted - r["st0"].accepted) / (r["region_ms"] / 1e3),
+        "ms_per_step": r["region_ms"] / steps, "stage_ms": r["stage_ms"],
+        "roofline_frac": alg / (r["stage_ms"] / 1e3) / HBM_PEAK,
+        "variant": {"rows_per_group": r["rows"], "nontemporal_loads": r["nt"], "chunk_ring": r["ring"]}}
+    name = "batch64-n20000-bf16"
+    b = batch_run(name, steps, warmup, proposer=False)
+    Nb = b["Ns"][0]
+    per_attempt = b["region_ms"] / steps
+    plain_ms = plain_attempt_ms(Nb)
+    alg_b = 2 * Nb * Nb + 2 * 2 * 128 * Nb
+    out[name] = {"batched_ms_per_attempt": per_attempt, "plain_fp32_ms_per_attempt": plain_ms,
+                 "instance_steps_speedup_vs_64_sequential_plain": 64 * plain_ms / per_attempt,
+                 "product_ms": b["coupling_ms"], "product_roofline_frac": alg_b / (b["coupling_ms"] / 1e3) / HBM_PEAK,
+                 "tile": {"workgroup_variant": b["tune"][0], "split_k": b["tune"][1]}}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -371,6 +406,8 @@ def main():
     ap.add_argument("--published", action="store_true",
                     help="run the reference's published command lines through the drop-in programs instead")
     ap.add_argument("--scratch", default=None, help="--published: directory for the programs' output files")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="default workload: skip the FP32 N=50000 and batched measurements that follow it")
     args = ap.parse_args()
     if args.published:
         published_main(args)
@@ -472,6 +509,14 @@ def main():
             out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_steps)
         except Exception as e:  # reported, never fatal for the GPU figure
             out["cpu_baseline"] = {"value": None, "error": repr(e)}
+    # after the CPU baseline: the GPU has idled for its duration, as before
+    # the headline run (the stage kernels stream 2-5% slower right after
+    # minutes of load)
+    if args.config == "fp64-n50000" and not args.no_secondary:
+        try:
+            out["secondary"] = secondary_measurements(args.steps, args.warmup)
+        except Exception as e:  # reported, never fatal for the headline figure
+            out["secondary"] = {"error": repr(e)}
     print(json.dumps(out), flush=True)
 
 
