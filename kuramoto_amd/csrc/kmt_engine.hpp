@@ -221,16 +221,17 @@ template <class R> struct Engine : EngineBase {
   // rows: each (cos, sin) chunk is reused for every row of the group from
   // registers, and at 8 rows the vectors' L2 bytes would match the 2-byte
   // entries' (FP64 state, BF16, N=50000: 0.754 against 0.811 ms per stage).
-  // A matrix in the working precision that streams from HBM takes the chunk
-  // ring (the same sums and grouping, 0.6% faster at N=50000 FP64 and FP32;
-  // slower with 16-bit entries at 16 rows, where it needs 231 registers).
+  // A matrix in the working precision takes the chunk ring (the same sums
+  // and grouping; 0.6% faster at N=50000 FP64 and FP32 from HBM, 4.4% at
+  // N=5000 FP32 from the Infinity Cache; slower with 16-bit entries at 16
+  // rows, where it needs 231 registers).
   // kmt_autotune / kmt_select_variant override it.
   void default_variant() {
     const bool cached = N >= 4096 && (double)N * (double)N * (double)elem_bytes(storage) <= 128e6;
     tune_rows = cached ? 4 : (elem_bytes(storage) == 2 ? 16 : 8);
     tune_nt = cached ? 0 : 1;
     tune_bal = 0;
-    tune_ring = !cached && elem_bytes(storage) != 2;
+    tune_ring = elem_bytes(storage) != 2;
     bal_grid = 0;
     tuned = 1;
     drop_graphs();
