@@ -16,6 +16,9 @@ the device and kept as packed signs (kmt_set_factor_sign), through the C ABI.
   max(1, |theta|), accepted steps within 2% (or 1), r(t) within 1e-3; the
   dense form of the same network (the matrix built from the signs) against
   the same oracle run: steps within 2% (or 1), r(t) within 1e-3.
+* At the largest published size, N = K = 10^5 (no oracle can hold that
+  factor), the symmetry invariants sum_i (sin t_i C_i - cos t_i S_i) = 0 and
+  sum_i f_i = sum_i w_i within 1e-8 (FP32) / 1e-14 (FP64) of the scale.
 """
 import numpy as np
 import pytest
@@ -107,3 +110,39 @@ def test_sign_factor_published_run_n10000_k10000(km, orc):
         assert abs(run["status"].accepted - ref["accepted"]) <= max(1, int(0.02 * ref["accepted"]))
         assert np.max(np.abs(run["r"] - ref["r"])) < 1e-3
     assert e1 < 1e-5
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_sign_factor_full_size_symmetry(km, prec):
+    """The reference's largest published -A network, N = K = 10^5
+    (volcano.ipynb:319-324), where the oracle cannot hold the factor (10^10
+    signs): M = X diag(w) X^T is symmetric, so sum_i (sin t_i C_i - cos t_i S_i)
+    = sum_ij M_ij sin(t_i - t_j) = 0 and sum_i f_i = sum_i w_i.  The packed
+    path sums exactly in integers and rounds C_i, S_i once, so the residual is
+    that rounding (and the device's cos / sin against the host's); a pass
+    that used the signs other than transposed to the first (an effective M
+    that is not symmetric) leaves a residual of order sum |C| / sqrt(N)."""
+    N = K = 100000
+    p = km.F32 if prec == "f32" else km.F64
+    dtype = np.float32 if prec == "f32" else np.float64
+    c1 = 3.0
+    rng = np.random.default_rng(11)
+    omega = rng.standard_normal(N).astype(dtype)
+    theta = rng.uniform(-30, 30, N).astype(dtype)
+    print(f"case sign factor full size N=K={N} {prec}", flush=True)
+    with km.Integrator(N, p, c1=c1) as it:
+        it.set_omega(omega)
+        it.set_factor_sign(K, 1)
+        it.use_factored(True)
+        f, cp = it.rhs(theta, couplings=True)
+    C, S = cp[0::2].astype(np.float64), cp[1::2].astype(np.float64)
+    t = theta.astype(np.float64)
+    scale = float(np.sum(np.abs(C)) + np.sum(np.abs(S)))
+    inv = float(np.sum(np.sin(t) * C - np.cos(t) * S))
+    # FP32: the float rounding of every C_i, S_i summed with random signs,
+    # ~6e-8 sqrt(N) |C| ~ 1e-10 of the scale (measured 1e-11); FP64 1e-17
+    tol = 1e-8 if prec == "f32" else 1e-14
+    print(f"  sum|C|+|S| = {scale:.4e}, antisymmetric sum = {inv:.3e}", flush=True)
+    assert scale > 0 and abs(inv) <= tol * scale
+    fsum = float(np.sum(f.astype(np.float64) - omega.astype(np.float64)))
+    assert abs(fsum) <= tol * (float(np.sum(np.abs(omega.astype(np.float64)))) + c1 * scale)
