@@ -225,15 +225,18 @@ int kmt_bandwidth_probe(int device, size_t bytes, int reps, double *copy_gbps, d
 
 /* Autotune the dense stage kernel on this context's adjacency: rows per
  * workgroup {4, 8, 16} x non-temporal loads {off, on} x grid {one workgroup
- * per row group, balanced (one resident round, equal contiguous row
- * ranges)}, `reps` timed launches each; keeps the fastest.  variant_ms (12
- * values, may be NULL) receives the average ms of each variant, index
- * 4*r + 2*nt + balanced with r = 0, 1, 2 for 4, 8, 16 rows.  Every row is
- * summed in the same order in every variant; the error norm's partial sums
- * follow the variant's workgroups, so its last bits (and h's) can differ
- * between variants.  Without this call kmt_begin uses a fixed choice (4 rows,
- * plain loads when N >= 4096 and the matrix is <= 128 MB; else 8 rows,
- * non-temporal), so that every context and process gives the same bits.
+ * per row group (timed with and without the chunk ring), balanced (one
+ * resident round, equal contiguous row ranges)}, `reps` timed launches
+ * each; keeps the fastest.  variant_ms (12 values, may be NULL) receives the
+ * average ms of each variant (the faster ring form for one workgroup per
+ * group), index 4*r + 2*nt + balanced with r = 0, 1, 2 for 4, 8, 16 rows.
+ * Every row is summed in the same order in every variant; the error norm's
+ * partial sums follow the variant's workgroups (the ring groups them as the
+ * loop without it does), so its last bits (and h's) can differ between
+ * variants.  Without this call kmt_begin uses a fixed choice (4 rows, plain
+ * loads when N >= 4096 and the matrix is <= 128 MB; else 8 rows, 16 for
+ * 16-bit entries, non-temporal; the chunk ring unless the entries are 16
+ * bits), so that every context and process gives the same bits.
  * Calling it (mid-run, or in only one of two contexts) therefore breaks
  * bit-exact restart against contexts that use the fixed choice.  KMT_ADJ_SYM has one
  * form and is not tuned.  No reference counterpart (cuBLAS chooses its own

@@ -244,3 +244,25 @@ def test_stage_ring_is_bit_identical(km, prec, storage, N):
             (f0, c0, g0), (f1, c1, g1) = out[False], out[True]
             assert np.array_equal(f0, f1) and np.array_equal(c0, c1), rows
             assert np.array_equal(g0["rows"], g1["rows"]) and np.array_equal(g0["times"], g1["times"]), rows
+
+
+def test_autotune_keeps_a_valid_variant(km):
+    """kmt_autotune times every variant (one workgroup per group with and
+    without the chunk ring) on the context's matrix and keeps the fastest:
+    the table is complete, the kept flags are a valid combination, and the
+    RHS is the same bits as under the fixed default."""
+    N = 3000
+    X = km.factor(N, 5, 31, gaussian=True).astype(np.float32)
+    w = km.factor_weights(km.F32, N, 5, gaussian=True)
+    th, om = km.draws(km.F32, N, 31, normal=True, init=6.283185307)
+    print("case autotune", flush=True)
+    with km.Integrator(N, km.F32, c1=3.0) as it:
+        it.set_omega(om)
+        it.set_factor(X, w)
+        it.build_adjacency()
+        f0 = it.rhs(th)
+        ms = it.autotune(3)
+        assert ms.shape == (3, 2, 2) and np.all(ms > 0) and np.all(ms < 1e3)
+        fl = it.variant_flags()
+        assert fl & ~7 == 0 and (fl & 6) != 6 and it.layout()[2] in (4, 8, 16)
+        assert np.array_equal(it.rhs(th), f0)
