@@ -91,20 +91,19 @@ template <class R> struct Engine : EngineBase {
   SymPair<double> *sym_prow = nullptr, *sym_pcol = nullptr;
   double *lr_part = nullptr, *lr_G = nullptr;
   int lr_nblk = 0, lr_grid = 0;
-  // packed +-1 factor (kmt_set_factor_sign): sign bits [N][s_ldw] and their
-  // transpose [32 s_kw][s_ldwT]; per pass (1: rows m over k = n, 2: rows n
-  // over k = m) the B fragments of the digits and the int32 partial sums
+  // packed +-1 factor (kmt_set_factor_sign): per pass (1: rows m over
+  // k = n, 2: rows n over k = m) the sign bits in the pass's tile order
+  // (ldp rows of 32 npairs words: the bit rows, the ones row, zero rows),
+  // the B fragments of the digits and the int32 partial sums
   int sign = 0;
-  uint32_t *sbits = nullptr, *sbitsT = nullptr;
-  long s_kw = 0, s_ldw = 0, s_nw = 0, s_ldwT = 0;
   struct SignPass {
     long ldp = 0, nsb = 0, npairs = 0, pairs_per = 1;  // nsb = 2 npairs super-blocks of 512
     int nsplit = 1;
+    uint32_t *tiles = nullptr;
     uint8_t *bfrag = nullptr;
     int32_t *part = nullptr;
     uint32_t *ones = nullptr;  // the ones row: bits set for k < kdim, 16 nsb words
   } s_pass[2];
-  uint32_t *s_zero = nullptr;  // a zero row of max(16 nsb) words
   std::vector<R> X_host, w_host;
 
   // run
@@ -513,11 +512,10 @@ template <class R> struct Engine : EngineBase {
   }
 
   void drop_sign() {
-    void *b[] = {sbits, sbitsT, s_zero, s_pass[0].bfrag, s_pass[0].part, s_pass[0].ones,
-                 s_pass[1].bfrag, s_pass[1].part, s_pass[1].ones};
+    void *b[] = {s_pass[0].tiles, s_pass[0].bfrag, s_pass[0].part, s_pass[0].ones,
+                 s_pass[1].tiles, s_pass[1].bfrag, s_pass[1].part, s_pass[1].ones};
     for (void *p : b)
       if (p) (void)hipFree(p);
-    sbits = sbitsT = s_zero = nullptr;
     s_pass[0] = s_pass[1] = SignPass();
     sign = 0;
   }
@@ -574,36 +572,40 @@ template <class R> struct Engine : EngineBase {
     for (int m = 0; m < K; ++m) w_host[m] = (R)(((m & 1) ? -1.0 : 1.0) / (double)N);
     HIP_TRY(hipMalloc((void **)&w, sizeof(R) * std::max(K, 1)));
     if (K > 0) HIP_TRY(hipMemcpyAsync(w, w_host.data(), sizeof(R) * K, hipMemcpyHostToDevice, stream));
-    s_kw = (K + 31) / 32;
-    s_ldw = round_up(std::max<long>(s_kw, 1), 32);  // whole super-block pairs of the pass-2 rows
-    s_nw = (N + 31) / 32;
-    s_ldwT = round_up(s_nw, 64);
-    hipError_t e = hipMalloc((void **)&sbits, sizeof(uint32_t) * (size_t)N * s_ldw);
-    if (e == hipSuccess) e = hipMalloc((void **)&sbitsT, sizeof(uint32_t) * (size_t)std::max<long>(32 * s_kw, 1) * s_ldwT);
+    s_pass[0] = sign_pass_shape(K, N);
+    s_pass[1] = sign_pass_shape(N, K);
+    SignPass &p1 = s_pass[0], &p2 = s_pass[1];
+    const size_t tw1 = (size_t)p1.ldp * 32 * p1.npairs, tw2 = (size_t)p2.ldp * 32 * p2.npairs;
+    hipError_t e = hipMalloc((void **)&p1.tiles, sizeof(uint32_t) * tw1);
+    if (e == hipSuccess) e = hipMalloc((void **)&p2.tiles, sizeof(uint32_t) * tw2);
     if (e != hipSuccess) {
       drop_sign();
       return fail(KMT_ERR_MEMORY, "cannot allocate the packed sign factor (N=%ld, K=%d): %s", N, K, hipGetErrorString(e));
     }
-    s_pass[0] = sign_pass_shape(K, N);
-    s_pass[1] = sign_pass_shape(N, K);
-    const long zwords = 16 * std::max(s_pass[0].nsb, s_pass[1].nsb);
-    if (alloc_sign_pass(s_pass[0], N) != KMT_OK || alloc_sign_pass(s_pass[1], K) != KMT_OK ||
-        hipMalloc((void **)&s_zero, sizeof(uint32_t) * zwords) != hipSuccess) {
+    if (alloc_sign_pass(p1, N) != KMT_OK || alloc_sign_pass(p2, K) != KMT_OK) {
       drop_sign();
       return fail(KMT_ERR_MEMORY, "cannot allocate the sign passes' buffers (N=%ld, K=%d)", N, K);
     }
-    HIP_TRY(hipMemsetAsync(s_zero, 0, sizeof(uint32_t) * zwords, stream));
-    // words of bitsT past N are never written by the transpose: zero them
-    HIP_TRY(hipMemsetAsync(sbitsT, 0, sizeof(uint32_t) * (size_t)std::max<long>(32 * s_kw, 1) * s_ldwT, stream));
-    const size_t words = (size_t)N * s_ldw;
-    gen_sign_bits_kernel<<<(unsigned)((words + kBlock - 1) / kBlock), kBlock, 0, stream>>>(sbits, N, K, s_ldw, seed);
+    // zero rows (and words past each row's k range) stay as cleared here
+    HIP_TRY(hipMemsetAsync(p1.tiles, 0, sizeof(uint32_t) * tw1, stream));
+    HIP_TRY(hipMemsetAsync(p2.tiles, 0, sizeof(uint32_t) * tw2, stream));
+    const size_t words = (size_t)((N + 63) / 64) * p2.npairs * 2048;
+    gen_sign_bits_kernel<<<(unsigned)((words + kBlock - 1) / kBlock), kBlock, 0, stream>>>(p2.tiles, N, K, p2.npairs,
+                                                                                          seed);
     HIP_TRY(hipGetLastError());
     if (K > 0) {
-      const size_t waves = (size_t)((N + 63) / 64) * s_kw;
+      const long kw = (K + 31) / 32;
+      const size_t waves = (size_t)((N + 63) / 64) * kw;
       transpose_sign_bits_kernel<<<(unsigned)((waves * kWave + kBlock - 1) / kBlock), kBlock, 0, stream>>>(
-          sbits, N, s_ldw, s_kw, sbitsT, s_ldwT);
+          p2.tiles, N, p2.npairs, kw, p1.tiles, p1.npairs);
       HIP_TRY(hipGetLastError());
     }
+    // the ones rows last: the transpose writes zero words into rows m >= K
+    sign_tile_ones_kernel<<<(unsigned)((32 * p1.npairs + kBlock - 1) / kBlock), kBlock, 0, stream>>>(
+        p1.tiles, K, p1.ones, p1.npairs);
+    sign_tile_ones_kernel<<<(unsigned)((32 * p2.npairs + kBlock - 1) / kBlock), kBlock, 0, stream>>>(
+        p2.tiles, N, p2.ones, p2.npairs);
+    HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(stream));
     sign = 1;
     return KMT_OK;
@@ -615,7 +617,7 @@ template <class R> struct Engine : EngineBase {
     for (long r0 = 0; r0 < N; r0 += chunk) {
       const long nr = std::min(chunk, N - r0);
       dim3 grid((unsigned)((N + kBlock - 1) / kBlock), (unsigned)((nr + ROWS - 1) / ROWS));
-      gen_adj_sign_kernel<R, S, ROWS><<<grid, kBlock, 0, stream>>>(sbits, s_ldw, w, K, N, r0, nr,
+      gen_adj_sign_kernel<R, S, ROWS><<<grid, kBlock, 0, stream>>>(s_pass[1].tiles, s_pass[1].npairs, w, K, N, r0, nr,
                                                                    (S *)A + (size_t)r0 * lda, lda, inv_scale);
       HIP_TRY(hipGetLastError());
     }
@@ -885,12 +887,12 @@ template <class R> struct Engine : EngineBase {
       sign_limbs_kernel<R><<<(unsigned)((n_pad + kBlock - 1) / kBlock), kBlock, 0, stream>>>(
           v.cs_in, v.cs_in + ldv, N, n_pad, p1.bfrag, st, chk);
       sign_mfma_kernel<R, kSignNT><<<(unsigned)b1, kBlock, 0, stream>>>(
-          sbitsT, s_ldwT, K, p1.ones, s_zero, (const u32x4 *)p1.bfrag, p1.npairs, p1.pairs_per, p1.nsplit,
+          (const u32x4 *)p1.tiles, (const u32x4 *)p1.bfrag, p1.npairs, p1.pairs_per, p1.nsplit,
           p1.part, p1.ldp, st, chk);
       sign_gram_kernel<R><<<(unsigned)((k_pad + kBlock - 1) / kBlock), kBlock, 0, stream>>>(
           p1.part, p1.nsplit, p1.ldp, w, K, k_pad, p2.bfrag, st, chk);
       sign_mfma_kernel<R, kSignNT><<<(unsigned)b2, kBlock, 0, stream>>>(
-          sbits, s_ldw, N, p2.ones, s_zero, (const u32x4 *)p2.bfrag, p2.npairs, p2.pairs_per, p2.nsplit, p2.part,
+          (const u32x4 *)p2.tiles, (const u32x4 *)p2.bfrag, p2.npairs, p2.pairs_per, p2.nsplit, p2.part,
           p2.ldp, st, chk);
     }
     sign_stage_kernel<R, KIND><<<(unsigned)((N + kBlock - 1) / kBlock), kBlock, 0, stream>>>(

@@ -450,7 +450,6 @@ __global__ __launch_bounds__(kBlock) void stage_ring_kernel(const DenseArgs<R, S
   constexpr long STRIDE = (long)kBlock * VEC;
   constexpr int WV = (int)(VEC * sizeof(R) / 16);  // 16-byte loads per vector chunk
   const int tid = threadIdx.x;
-  const long N = a.v.N;
   const long row0 = (long)blockIdx.x * ROWS;
   const long nch = (a.lda + STRIDE - 1) / STRIDE;
   __amdgpu_buffer_rsrc_t rrow[ROWS];

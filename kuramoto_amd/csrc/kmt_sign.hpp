@@ -4,22 +4,24 @@
 // counters inside m1 / m2 and never stores them; here they are drawn once on
 // the device from the library's own counter-based Philox stream (the same
 // bits as the host generator sign_factor in kmt_rng.h: element e = n K + m is
-// bit e mod 128 of block e / 128) and kept packed at one bit per entry, in
-// two layouts:
-//   bits  [N][ldw]      row n, word q: m = 32 q + b in bit b (zero past K)
-//   bitsT [32 Kw][ldwT] row m, word q: n = 32 q + b in bit b (zero past N)
-// (1.25 GB each at N = K = 10^5, against 40 GB for X in FP32).
+// bit e mod 128 of block e / 128) and kept packed at one bit per entry, once
+// per int8 pass, in that pass's tile order (sign_tile_word):
+//   pass 2 tiles: row n, word q: m = 32 q + b in bit b (zero past K)
+//   pass 1 tiles: row m, word q: n = 32 q + b in bit b (zero past N)
+// (1.25 GB each at N = K = 10^5, against 40 GB for X in FP32), each with one
+// more row of ones (k < the pass's k range) and zero rows up to its padded
+// row count.
 //
 // Coupling (one RHS, five launches), on the int8 matrix cores with exact
 // integer sums:
 //   sign_limbs_kernel  (cos, sin)_n as 56-bit fixed point (2^-54 units) in
 //                      7 signed base-256 digits each: the B operand of pass 1
-//   sign_mfma_kernel   pass 1 over bitsT: P_m = sum_n bit(m, n) * digit(n),
+//   sign_mfma_kernel   pass 1 over the pass-1 tiles: P_m = sum_n bit(m, n) * digit(n),
 //                      per digit and per range of n, plus T = sum_n digit(n)
 //   sign_gram_kernel   sum_n X_nm v_n = 2 P_m - T (X = 2 bit - 1) combined to
 //                      double, G_m = w_m * that, and G as digits: the B
 //                      operand of pass 2
-//   sign_mfma_kernel   pass 2 over bits: the same with rows n, k = m
+//   sign_mfma_kernel   pass 2 over the pass-2 tiles: the same with rows n, k = m
 //   sign_stage_kernel  C_n, S_n = 2 P_n - T combined, then the stage epilogue
 // Each digit's sum over k is exact in int32 (|sum| <= 128 k < 2^31 for
 // k < 2^24), so every coupling sum is the exact sum of the fixed-point
@@ -63,14 +65,32 @@ __device__ __forceinline__ uint32_t sign_stream_word(uint64_t W, uint32_t seed) 
   return sel == 0 ? x[0] : sel == 1 ? x[1] : sel == 2 ? x[2] : x[3];
 }
 
-// bits[n][q] = signs of m = 32 q .. 32 q + 31 (m < K) of row n: stream
-// elements n K + 32 q + b, i.e. 32 consecutive stream bits from position
-// e0 = n K + 32 q (at most two stream words).
-[[maybe_unused]] static __global__ __launch_bounds__(kBlock) void gen_sign_bits_kernel(uint32_t *bits, long N, long K, long ldw,
-                                                                uint32_t seed) {
-  const long idx = (long)blockIdx.x * kBlock + threadIdx.x;
-  if (idx >= N * ldw) return;
-  const long n = idx / ldw, q = idx % ldw;
+// Tile order of a pass's bits: rows in groups of 64 (one wave's four 16-row
+// MFMA tiles), words in pairs of 32 (1024 k, two super-blocks); the group's
+// 64 rows x 32 words of pair P are 8 KB of consecutive memory, in the order
+// one wave loads them: load (t, s) of lane (g, rl) is the 16 bytes of row
+// 16 t + rl, words 32 P + 8 g + 4 s .. + 3, at 16-byte slot
+// (2 t + s) 64 + 16 g + rl.  A wave thus streams one contiguous run as P
+// advances (row-major bits gave each lane's row 128 bytes every pair, rows
+// 12.5 KB apart at N = 10^5, and HBM served that at about half its rate).
+__host__ __device__ __forceinline__ size_t sign_tile_word(long r, long q, long npairs) {
+  const long rg = r >> 6, t = (r >> 4) & 3, rl = r & 15;
+  const long P = q >> 5, g = (q >> 3) & 3, s = (q >> 2) & 1, e = q & 3;
+  return ((((size_t)rg * npairs + P) * 8 + (2 * t + s)) * 64 + (16 * g + rl)) * 4 + e;
+}
+
+// Row n of the pass-2 tiles, word q: signs of m = 32 q .. 32 q + 31 (m < K):
+// stream elements n K + 32 q + b, i.e. 32 consecutive stream bits from
+// position e0 = n K + 32 q (at most two stream words).  One thread per word
+// of the rows n < N, in tile order (coalesced stores).
+[[maybe_unused]] static __global__ __launch_bounds__(kBlock) void gen_sign_bits_kernel(uint32_t *tiles, long N, long K,
+                                                                                      long npairs, uint32_t seed) {
+  const long o = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (o >= (N + 63) / 64 * npairs * 2048) return;
+  const long piece = o >> 2, lane = piece & 63, ts = (piece >> 6) & 7, tile = piece >> 9;
+  const long n = 64 * (tile / npairs) + 16 * (ts >> 1) + (lane & 15);
+  const long q = 32 * (tile % npairs) + 8 * (lane >> 4) + 4 * (ts & 1) + (o & 3);
+  if (n >= N) return;  // the ones row and zero rows are written separately
   uint32_t word = 0;
   if (32 * q < K) {
     const long nb = K - 32 * q < 32 ? K - 32 * q : 32;
@@ -80,20 +100,32 @@ __device__ __forceinline__ uint32_t sign_stream_word(uint64_t W, uint32_t seed) 
     if (sh != 0 && sh + nb > 32) word |= sign_stream_word((e0 >> 5) + 1, seed) << (32 - sh);
     if (nb < 32) word &= (1u << nb) - 1u;
   }
-  bits[idx] = word;
+  tiles[o] = word;
 }
 
-// bitsT from bits, 32 x 64 bit blocks: lane l holds row n0 + l's word q; the
-// ballot of bit b over the wave is column m = 32 q + b for those 64 rows,
-// i.e. two words of bitsT row m.  Lane b (< 32) writes row m = 32 q + b.
-[[maybe_unused]] static __global__ __launch_bounds__(kBlock) void transpose_sign_bits_kernel(const uint32_t *bits, long N, long ldw, long Kw,
-                                                                      uint32_t *bitsT, long ldwT) {
+// The ones row (row `rows`) of a pass's tiles: word q = ones[q].
+[[maybe_unused]] static __global__ __launch_bounds__(kBlock) void sign_tile_ones_kernel(uint32_t *tiles, long rows,
+                                                                                       const uint32_t *ones,
+                                                                                       long npairs) {
+  const long q = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (q < 32 * npairs) tiles[sign_tile_word(rows, q, npairs)] = ones[q];
+}
+
+// Pass-1 tiles from pass-2 tiles, 32 x 64 bit blocks: lane l holds row
+// n0 + l's word q; the ballot of bit b over the wave is column m = 32 q + b
+// for those 64 rows, i.e. words n0 / 32 and n0 / 32 + 1 of pass-1 row m.
+// Lane b (< 32) writes row m = 32 q + b (rows m >= K get zero words; the
+// ones row is written afterwards).
+[[maybe_unused]] static __global__ __launch_bounds__(kBlock) void transpose_sign_bits_kernel(const uint32_t *tiles2, long N,
+                                                                                            long npairs2, long Kw,
+                                                                                            uint32_t *tiles1,
+                                                                                            long npairs1) {
   const long wave = ((long)blockIdx.x * kBlock + threadIdx.x) / kWave;
   const int lane = threadIdx.x & (kWave - 1);
   const long nblk = (N + 63) / 64;
   if (wave >= nblk * Kw) return;  // whole waves: the ballot needs every lane
   const long q = wave / nblk, n0 = (wave % nblk) * 64;
-  const uint32_t w = (n0 + lane < N) ? bits[(n0 + lane) * ldw + q] : 0u;
+  const uint32_t w = (n0 + lane < N) ? tiles2[sign_tile_word(n0 + lane, q, npairs2)] : 0u;
   uint32_t lo = 0, hi = 0;
 #pragma unroll
   for (int b = 0; b < 32; ++b) {
@@ -104,9 +136,9 @@ __device__ __forceinline__ uint32_t sign_stream_word(uint64_t W, uint32_t seed) 
     }
   }
   if (lane < 32) {
-    uint32_t *dst = bitsT + (32 * q + lane) * ldwT + n0 / 32;
-    dst[0] = lo;
-    if (n0 / 32 + 1 < ldwT) dst[1] = hi;
+    const long m = 32 * q + lane, c = n0 / 32;
+    tiles1[sign_tile_word(m, c, npairs1)] = lo;
+    if (c + 1 < 32 * npairs1) tiles1[sign_tile_word(m, c + 1, npairs1)] = hi;
   }
 }
 
@@ -118,10 +150,11 @@ __device__ __forceinline__ uint32_t sign_stream_word(uint64_t W, uint32_t seed) 
 // columns: the 7 digits of the cos (or G_c) value in 0..6, of sin (G_s) in
 // 7..13, zero in 14 and 15.
 //
-// k is walked in pairs of 512-wide super-blocks (32 bit words = one 128-byte
-// line per row).  Lane (row, g) loads words 32 P + 8 g .. + 7 of its row as
-// two 16-byte vectors, the 4 lanes of a row one whole line; super-block
-// S = 2 P + s takes words 32 P + 8 g + 4 s + u, u = 0..3.  For word u and
+// k is walked in pairs of 512-wide super-blocks (32 bit words = 128 bytes
+// per row).  Lane (row, g) loads words 32 P + 8 g .. + 7 of its row as two
+// 16-byte vectors from the pass's tiles (sign_tile_word: the wave's 8 KB of
+// the pair are consecutive); super-block S = 2 P + s takes words
+// 32 P + 8 g + 4 s + u, u = 0..3.  For word u and
 // half h (0, 1) the A operand is the four dwords (word >> (4 h + j)) &
 // 0x01010101, j = 0..3, so byte i of dword j is bit 8 i + 4 h + j:
 // k = 32 (32 P + 8 g + 4 s + u) + 8 i + 4 h + j.  The B operand of (S, u, h)
@@ -131,8 +164,8 @@ __device__ __forceinline__ uint32_t sign_stream_word(uint64_t W, uint32_t seed) 
 // ---------------------------------------------------------------------
 typedef int i32x4_t __attribute__((ext_vector_type(4)));
 // Register sets of the pass's ring: one pair in use, kSignPF - 1 in flight.
-// 2 sets keep 3 waves per SIMD (132 VGPRs); 3 sets (182 VGPRs, 2 waves) ran
-// 5% slower at N = K = 10^5, 4 sets slower still.
+// 2 sets keep 4 waves per SIMD (124 VGPRs); 3 sets (2 waves per SIMD, the
+// same bytes in flight per SIMD) ran 4% slower at N = K = 10^5.
 constexpr int kSignPF = 2;
 
 template <int I, int U, class F> __device__ __forceinline__ void sign_unroll(F &&f) {
@@ -199,9 +232,8 @@ __device__ __forceinline__ i32x4_t sign_expand(uint32_t w, int h) {
 }
 
 // One pass: part[sp][r][c] = sum over the k of split sp of bit(r, k) * B[k][c]
-// for rows r < rows (bit rows of `bits`, ldw words each, a multiple of 32),
-// row `rows` = ones_row (ones for k < kdim: T_c = sum_k B[k][c]), rows
-// beyond: zero_row (both ldw words).  A workgroup = 4 waves on the same
+// for the rows r < ldp of the pass's tiles (its bit rows, then the ones row:
+// T_c = sum_k B[k][c], then zero rows).  A workgroup = 4 waves on the same
 // split of super-block pairs; wave w owns row group 4 (group of blocks) + w,
 // NT 16-row tiles.  The B fragments of a pair (16 KB) are loaded once per
 // workgroup, 64 bytes per thread, and shared through LDS (double-buffered,
@@ -210,9 +242,7 @@ __device__ __forceinline__ i32x4_t sign_expand(uint32_t w, int h) {
 // after them) are issued in the order they are used, so kSignPF - 1 pairs
 // stay in flight behind the one in use.
 template <class R, int NT>
-__global__ __launch_bounds__(kBlock) void sign_mfma_kernel(const uint32_t *__restrict__ bits, long ldw, long rows,
-                                                            const uint32_t *__restrict__ ones_row,
-                                                            const uint32_t *__restrict__ zero_row,
+__global__ __launch_bounds__(kBlock) void sign_mfma_kernel(const u32x4 *__restrict__ tiles,
                                                             const u32x4 *__restrict__ bfrag, long npairs,
                                                             long pairs_per, int nsplit, int32_t *__restrict__ part,
                                                             long ldp, const State<R> *st, int check_state) {
@@ -226,15 +256,11 @@ __global__ __launch_bounds__(kBlock) void sign_mfma_kernel(const uint32_t *__res
   const long rg = rgb * NW + wave;
   const long p0 = (long)sp * pairs_per, p1 = p0 + pairs_per < npairs ? p0 + pairs_per : npairs;
   const int g = lane >> 4, rl = lane & 15;
-  // Every lane loads (no divergent branch around the loads, which would make
-  // the compiler wait for every load in flight): the ones and zero rows are
-  // real rows in memory.
-  const uint32_t *rowp[NT];
-#pragma unroll
-  for (int t = 0; t < NT; ++t) {
-    const long r = rg * 16 * NT + 16 * t + rl;
-    rowp[t] = (r < rows ? bits + r * ldw : (r == rows ? ones_row : zero_row)) + 8 * g;
-  }
+  static_assert(NT == 4, "tile order: 64 rows per wave");
+  // the wave's 8 KB of pair P start at tw + 512 P (16-byte slots); every
+  // lane loads (no divergent branch around the loads, which would make the
+  // compiler wait for every load in flight)
+  const u32x4 *tw = tiles + (size_t)rg * npairs * 512 + lane;
   i32x4_t acc[NT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) acc[t] = i32x4_t{0, 0, 0, 0};
@@ -244,8 +270,8 @@ __global__ __launch_bounds__(kBlock) void sign_mfma_kernel(const uint32_t *__res
   auto load_w = [&](long P, u32x4 (&W)[NT][2]) {
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-      W[t][0] = *reinterpret_cast<const u32x4 *>(rowp[t] + 32 * P);
-      W[t][1] = *reinterpret_cast<const u32x4 *>(rowp[t] + 32 * P + 4);
+      W[t][0] = tw[(size_t)P * 512 + (2 * t) * 64];
+      W[t][1] = tw[(size_t)P * 512 + (2 * t + 1) * 64];
     }
   };
   auto load_b = [&](long P, u32x4 (&Bst)[BPT]) {
@@ -375,10 +401,10 @@ __global__ __launch_bounds__(kBlock) void sign_stage_kernel(const int32_t *__res
 // (gen_adj_kernel): M_ij = sum_m fma(w_m, X_im X_jm, acc) in the working
 // precision, X_im X_jm = +1 where the two bits agree, -1 where they differ.
 // A workgroup owns ROWS rows and 256 columns; each thread reads its column's
-// bit row one word (32 m) at a time, the rows' words are the same in every
-// lane.
+// bit row (from the pass-2 tiles) one word (32 m) at a time, the rows'
+// words are the same in every lane.
 template <class R, class S, int ROWS>
-__global__ __launch_bounds__(kBlock) void gen_adj_sign_kernel(const uint32_t *bits, long ldw, const R *w, long K,
+__global__ __launch_bounds__(kBlock) void gen_adj_sign_kernel(const uint32_t *tiles, long npairs, const R *w, long K,
                                                                long N, long row0, long nrows, S *A, long lda,
                                                                R inv_scale) {
   const long i0 = row0 + (long)blockIdx.y * ROWS;
@@ -387,15 +413,14 @@ __global__ __launch_bounds__(kBlock) void gen_adj_sign_kernel(const uint32_t *bi
   R acc[ROWS];
 #pragma unroll
   for (int r = 0; r < ROWS; ++r) acc[r] = (R)0;
-  const uint32_t *bj = bits + jj * ldw;
   const long kw = (K + 31) / 32;
   for (long q = 0; q < kw; ++q) {
-    const uint32_t wj = bj[q];
+    const uint32_t wj = tiles[sign_tile_word(jj, q, npairs)];
     uint32_t wi[ROWS];
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) {
       const long i = i0 + r < N ? i0 + r : N - 1;
-      wi[r] = bits[i * ldw + q];
+      wi[r] = tiles[sign_tile_word(i, q, npairs)];
     }
     const int nb = K - 32 * q < 32 ? (int)(K - 32 * q) : 32;
     for (int b = 0; b < nb; ++b) {
