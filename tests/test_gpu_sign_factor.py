@@ -3,8 +3,9 @@ the device and kept as packed signs (kmt_set_factor_sign), through the C ABI.
 
 * The coupling through the packed signs against the oracle's factored path
   (kuramoto.cu:62-101 restated: G_m = w_m sum_n X_nm v_n, C_n = sum_m X_nm G_m)
-  on the host generator's signs (kmt_factor(N, K, seed, 0)), K = 17 .. 5000
-  and K > N: FP32 bit-identical in >= 99% of the values and within 2.5e-7 of
+  on the host generator's signs (kmt_factor(N, K, seed, 0)), K = 17 .. 5000,
+  K > N, K = 0, a single row, and sizes on the 64-row / 1024-column tile
+  boundaries of the passes' bit layout: FP32 bit-identical in >= 99% of the values and within 2.5e-7 of
   the row scale |w_i| + c K everywhere (both sides sum in double and round
   once); FP64 within 1e-13 of that scale.
 * The dense matrix built from the packed signs equals the oracle generator's
@@ -31,7 +32,7 @@ def _phase_err(a, b):
     return float(np.max(np.abs(np.asarray(a, np.float64) - b64) / np.maximum(1.0, np.abs(b64))))
 
 
-@pytest.mark.parametrize("N,K", [(1000, 17), (3001, 300), (257, 1000), (5000, 5000)])
+@pytest.mark.parametrize("N,K", [(1000, 17), (3001, 300), (257, 1000), (5000, 5000), (1, 2049), (64, 1024), (130, 0)])
 @pytest.mark.parametrize("prec", ["f32", "f64"])
 def test_sign_factor_rhs_matches_oracle(km, orc, N, K, prec):
     dtype = np.float32 if prec == "f32" else np.float64
