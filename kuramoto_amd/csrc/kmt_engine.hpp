@@ -524,18 +524,29 @@ template <class R> struct Engine : EngineBase {
   static constexpr int kSignNT = 4;
   // Shape of a pass with `rows` output rows (+ the ones row) over kdim:
   // workgroups of 4 row groups of 16 kSignNT rows x splits of the 1024-wide
-  // super-block pairs, about 8192 waves (2.7 rounds of the 3 waves per SIMD
-  // the kernel's registers allow, so the last round's tail is short; 12288
-  // measured the same) with at least 2 pairs per split.
+  // super-block pairs, at least 2 pairs per split.  The split count keeps
+  // the waves within two rounds of the `resident` waves the device holds at
+  // once and, among those counts, fills the last round best: a wave count
+  // just past a whole round leaves most of the GPU idle behind a few waves.
+  // N = K = 10^5 on 256 CUs at 3 workgroups per CU (3072 resident waves):
+  // 3 splits = 4692 waves = 1.53 rounds, 0.514-0.532 ms per RHS; 2 splits
+  // (1.02 rounds) 0.577, 4 (2.04) 0.551, 5 (2.55) 0.535, 6 (3.05) 0.563,
+  // 7 (3.56) 0.551; fewer splits also write and re-read fewer partials.
   static constexpr long kSignRowsPerBlock = 16 * kSignNT * (kBlock / kWave);
-  static SignPass sign_pass_shape(long rows, long kdim) {
+  static SignPass sign_pass_shape(long rows, long kdim, long resident) {
     SignPass p;
     p.ldp = (rows + 1 + kSignRowsPerBlock - 1) / kSignRowsPerBlock * kSignRowsPerBlock;
     const long rg = p.ldp / (16 * kSignNT);
     p.npairs = std::max<long>(1, (kdim + 2 * kSignSB - 1) / (2 * kSignSB));
     p.nsb = 2 * p.npairs;
-    long sp = std::max<long>(1, (8192 + rg - 1) / rg);
-    sp = std::min<long>(sp, std::max<long>(1, p.npairs / 2));
+    const long spmax = std::min<long>(std::max<long>(1, 2 * resident / rg), std::max<long>(1, p.npairs / 2));
+    long sp = 1;
+    double best = -1.0;
+    for (long c = 1; c <= spmax; ++c) {
+      const long w = rg * c, rounds = (w + resident - 1) / resident;
+      const double fill = (double)w / (double)(rounds * resident);
+      if (fill > best) best = fill, sp = c;
+    }
     p.pairs_per = (p.npairs + sp - 1) / sp;
     p.nsplit = (int)((p.npairs + p.pairs_per - 1) / p.pairs_per);
     return p;
@@ -572,8 +583,14 @@ template <class R> struct Engine : EngineBase {
     for (int m = 0; m < K; ++m) w_host[m] = (R)(((m & 1) ? -1.0 : 1.0) / (double)N);
     HIP_TRY(hipMalloc((void **)&w, sizeof(R) * std::max(K, 1)));
     if (K > 0) HIP_TRY(hipMemcpyAsync(w, w_host.data(), sizeof(R) * K, hipMemcpyHostToDevice, stream));
-    s_pass[0] = sign_pass_shape(K, N);
-    s_pass[1] = sign_pass_shape(N, K);
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sign_mfma_kernel<R, kSignNT>, kBlock, 0) != hipSuccess ||
+        per_cu < 1)
+      per_cu = 1;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus < 1) cus = 1;
+    const long resident = (long)per_cu * cus * (kBlock / kWave);
+    s_pass[0] = sign_pass_shape(K, N, resident);
+    s_pass[1] = sign_pass_shape(N, K, resident);
     SignPass &p1 = s_pass[0], &p2 = s_pass[1];
     const size_t tw1 = (size_t)p1.ldp * 32 * p1.npairs, tw2 = (size_t)p2.ldp * 32 * p2.npairs;
     hipError_t e = hipMalloc((void **)&p1.tiles, sizeof(uint32_t) * tw1);

@@ -164,8 +164,9 @@ __host__ __device__ __forceinline__ size_t sign_tile_word(long r, long q, long n
 // ---------------------------------------------------------------------
 typedef int i32x4_t __attribute__((ext_vector_type(4)));
 // Register sets of the pass's ring: one pair in use, kSignPF - 1 in flight.
-// 2 sets keep 4 waves per SIMD (124 VGPRs); 3 sets (2 waves per SIMD, the
-// same bytes in flight per SIMD) ran 4% slower at N = K = 10^5.
+// 2 sets: 124 VGPRs (the occupancy query reports 3 workgroups per CU); 3
+// sets (fewer waves per SIMD, about the same bytes in flight per SIMD) ran
+// 4% slower at N = K = 10^5.
 constexpr int kSignPF = 2;
 
 template <int I, int U, class F> __device__ __forceinline__ void sign_unroll(F &&f) {
