@@ -358,24 +358,43 @@ def published_main(args):
                       "runs": rows}), flush=True)
 
 
-def secondary_measurements(steps, warmup):
-    """Two more workloads measured in the same process after the headline
-    one, so that one bench run also covers them: the FP32 N=50000 dense run
-    (the plain integrator's target configuration, >= 85% of 8 TB/s per RHS)
-    and the 64-instance shared-adjacency batch (BF16 A, MFMA product) per
-    attempt against 64 plain FP32 attempts of the same size.  No whole runs
-    to a fixed time here (those take minutes; --config batch64-n20000-bf16
-    reports them)."""
-    out = {}
-    cfg = CONFIGS["fp32-n50000"]
-    r = gpu_run(cfg, steps, warmup)
+def rhs_bytes(cfg, r):
+    """Algorithmic bytes of one RHS of a single-context workload."""
     N = r["N"]
-    alg = r["elem_bytes"] * N * N + N * 3 * 4
-    out["fp32-n50000"] = {
-        "value_osc_steps_per_s": N * (r["st"].accepted - r["st0"].accepted) / (r["region_ms"] / 1e3),
-        "ms_per_step": r["region_ms"] / steps, "stage_ms": r["stage_ms"],
-        "roofline_frac": alg / (r["stage_ms"] / 1e3) / HBM_PEAK,
-        "variant": {"rows_per_group": r["rows"], "nontemporal_loads": r["nt"], "chunk_ring": r["ring"]}}
+    s_y = np.dtype(r["dtype"]).itemsize
+    if cfg["storage"] == "factored":
+        # per RHS: X read twice (N K s_y each) + theta-derived vectors in, k out, omega
+        return 2 * N * cfg["K"] * s_y + N * (2 * s_y + s_y)
+    if cfg["storage"] == "signs":
+        # per RHS: the packed signs read twice (N K / 8 bytes each pass)
+        return 2 * N * cfg["K"] // 8 + N * (2 * s_y + s_y)
+    if cfg["storage"].startswith("sym"):
+        # a symmetric matrix carries N (N + 1) / 2 distinct entries
+        return r["elem_bytes"] * N * (N + 1) // 2 + N * (2 * s_y + s_y)
+    return r["elem_bytes"] * N * N + N * (2 * s_y + s_y)
+
+
+def secondary_measurements(steps, warmup):
+    """More workloads measured in the same process after the headline one,
+    so that one bench run also covers them: the FP32 N=50000 dense run (the
+    plain integrator's target configuration, >= 85% of 8 TB/s per RHS), the
+    FP64 N=50000 run on symmetric storage (upper block triangle), the
+    reference's high-rank -A network at N = K = 10^5 (packed signs, int8
+    passes), and the 64-instance shared-adjacency batch (BF16 A, MFMA
+    product) per attempt against 64 plain FP32 attempts of the same size.
+    No whole runs to a fixed time here (those take minutes;
+    --config batch64-n20000-bf16 reports them)."""
+    out = {}
+    for name in ("fp32-n50000", "fp64-n50000-sym", "fp32-n100000-k100000-signs"):
+        cfg = CONFIGS[name]
+        r = gpu_run(cfg, steps, warmup)
+        N = r["N"]
+        out[name] = {
+            "value_osc_steps_per_s": N * (r["st"].accepted - r["st0"].accepted) / (r["region_ms"] / 1e3),
+            "ms_per_step": r["region_ms"] / steps, "stage_ms": r["stage_ms"],
+            "roofline_frac": rhs_bytes(cfg, r) / (r["stage_ms"] / 1e3) / HBM_PEAK}
+        if cfg["storage"] not in ("sym", "signs"):
+            out[name]["variant"] = {"rows_per_group": r["rows"], "nontemporal_loads": r["nt"], "chunk_ring": r["ring"]}
     name = "batch64-n20000-bf16"
     b = batch_run(name, steps, warmup, proposer=False)
     Nb = b["Ns"][0]
@@ -407,7 +426,8 @@ def main():
                     help="run the reference's published command lines through the drop-in programs instead")
     ap.add_argument("--scratch", default=None, help="--published: directory for the programs' output files")
     ap.add_argument("--no-secondary", action="store_true",
-                    help="default workload: skip the FP32 N=50000 and batched measurements that follow it")
+                    help="default workload: skip the FP32 N=50000, symmetric, packed-sign and batched "
+                         "measurements that follow it")
     args = ap.parse_args()
     if args.published:
         published_main(args)
@@ -422,20 +442,9 @@ def main():
     cfg = CONFIGS[args.config]
     r = gpu_run(cfg, args.steps, args.warmup)
     N = r["N"]
-    s_y = np.dtype(r["dtype"]).itemsize
     accepted = r["st"].accepted - r["st0"].accepted
     secs = r["region_ms"] / 1e3
-    if cfg["storage"] == "factored":
-        # per RHS: X read twice (N K s_y each) + theta-derived vectors in, k out, omega
-        alg_bytes = 2 * N * cfg["K"] * s_y + N * (2 * s_y + s_y)
-    elif cfg["storage"] == "signs":
-        # per RHS: the packed signs read twice (N K / 8 bytes each pass)
-        alg_bytes = 2 * N * cfg["K"] // 8 + N * (2 * s_y + s_y)
-    elif cfg["storage"].startswith("sym"):
-        # a symmetric matrix carries N (N + 1) / 2 distinct entries
-        alg_bytes = r["elem_bytes"] * N * (N + 1) // 2 + N * (2 * s_y + s_y)
-    else:
-        alg_bytes = r["elem_bytes"] * N * N + N * (2 * s_y + s_y)
+    alg_bytes = rhs_bytes(cfg, r)
     achieved = alg_bytes / (r["stage_ms"] / 1e3)
     roof = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
             "frac": achieved / HBM_PEAK}
@@ -469,7 +478,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if s_y == 8 else "f32",
+        "dtype": "f64" if np.dtype(r["dtype"]).itemsize == 8 else "f32",
         "data": "synthetic (Philox Gaussian factor, glibc-rand frequencies/phases, as the CLI generates)",
         "config": {"workload": f"{args.config}: N={N}, K={cfg['K']}, "
                                + ("low-rank factor only (-A)" if cfg["storage"] == "factored" else
